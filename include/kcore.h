@@ -1,0 +1,132 @@
+/*
+ * kcore.h — C ABI of libkcore_hip.so, the MI355X (gfx950) k-core peeling engine.
+ *
+ * The reference exposes its GPU path as nine `int kcoreX(Graph&)` host functions
+ * (src/ours.cc:117-160, src/ours-shared.cc:119, src/ours-prefetch.cc:153, src/ballot*.cc,
+ * src/efficient*.cc), each doing cudaMalloc + H2D of the CSR (src/gpu_memory_allocation.cu:4-12),
+ * the level loop, and returning elapsed ms, with core numbers left on the device
+ * (get_results_from_gpu, src/gpu_memory_allocation.cu:14-16, never called). This ABI splits
+ * that into a resident device graph and a peel call that returns core numbers:
+ *
+ *   kc_open / kc_close           <- (implicit CUDA context of the reference)
+ *   kc_graph_upload              <- malloc_graph_gpu_memory   src/gpu_memory_allocation.cu:4-12
+ *   kc_graph_free                <- free_graph_gpu_memory     src/gpu_memory_allocation.cu:18-22
+ *   kc_peel                      <- kcore / kcoreSharedMem / kcorePrefetch (+ BC/EC clones)
+ *                                   src/ours.cc:117-160 (variant chosen by kc_params.variant)
+ *   kc_graph_download_core       <- get_results_from_gpu      src/gpu_memory_allocation.cu:14-16
+ *   kc_graph_download_order      <- KCore glBuffer + cShellOffsets (peel order, k-shell starts)
+ *                                   kcore-global-buffer/KCore.h:244-272
+ *   kc_batch_*                   <- (no reference counterpart: batched peeling of many graphs)
+ *   kc_serve_*                   <- (no reference counterpart: continuous-batching server)
+ *   kc_status_str                <- chkerr's message print   inc/gpu_memory_allocation.h:9-16
+ *
+ * Conventions: the caller owns host buffers; the library owns device state; handles are
+ * opaque; no function exits the process; every call returns a kc_status. A peel never
+ * modifies the uploaded CSR or degrees (it peels a working copy), so repeated peels and
+ * variant comparisons start from the same input. One kc_ctx per host thread.
+ */
+#ifndef KCORE_H
+#define KCORE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int kc_status;
+enum {
+    KC_OK = 0,
+    KC_ERR_ARG = 1,         /* invalid argument or graph shape */
+    KC_ERR_HIP = 2,         /* a HIP runtime call failed */
+    KC_ERR_OOM = 3,         /* device allocation failed */
+    KC_ERR_CAPACITY = 4,    /* request beyond a documented maximum */
+    KC_ERR_NO_PROGRESS = 5, /* level loop passed the maximum degree with vertices left */
+    KC_ERR_DEVICE = 6,      /* a device-side wait exceeded its time limit */
+    KC_ERR_NO_DEVICE = 7    /* no usable GPU */
+};
+
+typedef struct kc_ctx kc_ctx;
+typedef struct kc_graph kc_graph;
+
+/* Host CSR as the reference's Graph holds it (inc/graph.h:9-15, G_pointers inc/common.h:45-50):
+ * offsets[V+1], neighbors[arcs] (ascending, duplicate-free, symmetric), degrees[V]
+ * (may be NULL: taken from offsets). arcs must be < 2^32. */
+typedef struct {
+    uint32_t V;
+    uint64_t arcs;
+    const uint32_t* offsets;
+    const uint32_t* neighbors;
+    const uint32_t* degrees;
+} kc_csr;
+
+/* Peel variants. Every variant produces identical core numbers; they differ in how the
+ * frontier is staged. The reference's labels map as: Ours -> WAVE, SM -> LDS, VP -> PREFETCH
+ * (main.cu:60-88); its BC/EC compaction flavours are the wave64 ballot compaction used by all. */
+enum {
+    KC_VARIANT_AUTO = 0,
+    KC_VARIANT_WAVE = 1,     /* one persistent kernel; each wave drains global frontier tickets */
+    KC_VARIANT_LDS = 2,      /* + workgroup LDS frontier buffer for cascades, spilling to global */
+    KC_VARIANT_PREFETCH = 3, /* + next-batch adjacency prefetch while the current batch runs */
+    KC_VARIANT_STEPWISE = 4  /* reference-shaped: select and process kernels launched per level */
+};
+
+typedef struct {
+    int variant;              /* KC_VARIANT_* */
+    uint32_t block;           /* threads per workgroup (256, 512 or 1024); 0 = tuned default */
+    uint32_t lds_depth;       /* LDS frontier entries per workgroup (LDS variant); 0 = default */
+    uint32_t heavy_threshold; /* degree above which a vertex's arcs are split into chunks shared
+                                 by many waves; 0 = default, UINT32_MAX = never split */
+    uint32_t flags;           /* KC_FLAG_* */
+} kc_params;
+
+enum {
+    KC_FLAG_NO_LEVEL_SKIP = 1u << 0 /* scan every level 0..kmax like the reference, even empty ones */
+};
+
+typedef struct {
+    double ms_peel;           /* device time from "CSR + degrees resident" to "cores + kmax resident" */
+    double ms_h2d;            /* upload time of the graph (set by kc_graph_upload's caller stats) */
+    double ms_d2h;            /* core download time when kc_peel copies to the host */
+    double ms_kernel;         /* device time of the peeling kernel(s) alone, without state reset */
+    uint32_t kmax;
+    uint32_t levels;          /* levels whose degree==level scan ran */
+    uint64_t scanned_entries; /* degree entries read by the select scans */
+    uint64_t bytes_alg;       /* algorithmic bytes: 4*scanned + 16*n + 20*m */
+    uint64_t bytes_scan;      /* 4*scanned_entries */
+    uint64_t bytes_edge;      /* 16*n + 20*m */
+    uint32_t grid;            /* workgroups launched */
+    uint32_t block;           /* threads per workgroup */
+} kc_stats;
+
+const char* kc_status_str(kc_status s);
+
+kc_status kc_open(int device, kc_ctx** out);
+void kc_close(kc_ctx* ctx);
+
+/* Copies the CSR to the device and allocates the peel state for it. */
+kc_status kc_graph_upload(kc_ctx* ctx, const kc_csr* g, kc_graph** out);
+void kc_graph_free(kc_graph* g);
+void kc_graph_info(const kc_graph* g, uint32_t* V, uint64_t* arcs, uint32_t* dmax);
+
+/* Peels the graph. core_out (host, V entries) and kmax_out / stats may be NULL; the core
+ * numbers also stay on the device until the next peel of the same graph. */
+kc_status kc_peel(kc_ctx* ctx, kc_graph* g, const kc_params* p, uint32_t* core_out,
+                  uint32_t* kmax_out, kc_stats* stats);
+
+/* Results of the last peel: core numbers (V entries), peel order (V entries: vertices in the
+ * order they were peeled) and k-shell starts (shell_off[k] = first position in the order of a
+ * vertex with core k, for k = 0..kmax+1; kmax+2 entries). */
+kc_status kc_graph_download_core(kc_ctx* ctx, const kc_graph* g, uint32_t* core_out);
+kc_status kc_graph_download_order(kc_ctx* ctx, const kc_graph* g, uint32_t* order_out,
+                                  uint32_t* shell_off_out);
+/* Device pointer of the core array (valid until kc_graph_free), for zero-copy consumers. */
+const uint32_t* kc_graph_core_device(const kc_graph* g);
+
+/* Diagnostics (no reference counterpart): checks the wave64 DPP prefix sum the kernels use
+ * against a host prefix sum on one wavefront. KC_OK when they agree. */
+kc_status kc_selftest(kc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KCORE_H */

@@ -1,0 +1,90 @@
+/*
+ * kcore_graph.h — C ABI of libkcore_host.so: the edge-list -> CSR loader with the
+ * reference's semantics, its serialized cache and core-number writers, and the
+ * seeded synthetic graph generators used by the tests and the benchmark.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   kcg_load_edgelist      <- Graph::Graph / Graph::readFile        src/graph.cpp:58-124,142-154
+ *   kcg_read_serialized    <- Graph::readSerialized                 src/graph.cpp:9-33
+ *   kcg_write_serialized   <- Graph::writeSerialized                src/graph.cpp:35-54
+ *   kcg_write_kcore_json   <- Graph::writeKCoreToDisk               src/graph.cpp:126-140
+ *   kcg_graph_arrays       <- the public fields of class Graph      inc/graph.h:7-23
+ * The generators have no reference counterpart (the reference ships no inputs);
+ * they produce the same kind of input the reference's loader builds.
+ *
+ * Ownership: every kcg_graph* is owned by the library and released with
+ * kcg_graph_free. Array pointers returned by kcg_graph_arrays stay valid until then.
+ * No function calls exit(); each returns a status (0 = ok).
+ */
+#ifndef KCORE_GRAPH_H
+#define KCORE_GRAPH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kcg_graph kcg_graph;
+
+enum {
+    KCG_OK = 0,
+    KCG_ERR_IO = 1,       /* file missing / unwritable */
+    KCG_ERR_RANGE = 2,    /* vertex id or arc count beyond uint32 */
+    KCG_ERR_ARG = 3,      /* bad argument */
+    KCG_ERR_FORMAT = 4    /* malformed serialized cache */
+};
+
+/* Load a whitespace-separated 'u v' edge list exactly as the reference loader does:
+ * lines starting with an alphabetic char are skipped, self loops dropped,
+ * V = 1 + max id seen, symmetrised, deduplicated, ascending adjacency.
+ * nthreads <= 0 uses all host threads. */
+int kcg_load_edgelist(const char* path, int nthreads, kcg_graph** out);
+
+/* Serialized CSR cache in the reference's byte format ("V\nE\n" + degrees + offsets + neighbours). */
+int kcg_read_serialized(const char* path, kcg_graph** out);
+int kcg_write_serialized(const kcg_graph* g, const char* path);
+
+/* Reference constructor behaviour: try <outdir>/serialized-<basename(path)> first, else parse
+ * the text file and write that cache. *from_cache is set to 1 when the cache was used. */
+int kcg_load_cached(const char* path, const char* outdir, int nthreads, kcg_graph** out, int* from_cache);
+
+/* Build a CSR from an explicit pair list (same rules as the text loader). */
+int kcg_from_pairs(const uint32_t* src, const uint32_t* dst, uint64_t npairs, int nthreads, kcg_graph** out);
+
+/* Wrap a caller-provided CSR (copied). */
+int kcg_from_csr(uint32_t V, uint64_t arcs, const uint32_t* offsets, const uint32_t* neighbors, kcg_graph** out);
+
+void kcg_graph_free(kcg_graph* g);
+void kcg_graph_info(const kcg_graph* g, uint32_t* V, uint64_t* arcs, uint32_t* dmax);
+/* Pointers into library memory: offsets[V+1], neighbors[arcs], degrees[V]. */
+void kcg_graph_arrays(const kcg_graph* g, const uint32_t** offsets, const uint32_t** neighbors, const uint32_t** degrees);
+
+/* JSON dictionary of non-zero core numbers, byte-identical to writeKCoreToDisk. */
+int kcg_write_kcore_json(const char* path, uint32_t V, const uint32_t* core);
+/* Raw little-endian uint32[V] core dump. */
+int kcg_write_core_u32(const char* path, uint32_t V, const uint32_t* core);
+
+/* ---- seeded synthetic generators (deterministic for a given seed, any thread count) ---- */
+
+/* Erdos-Renyi G(n, m): the first m distinct non-loop pairs of a seeded uniform stream. */
+int kcg_gen_er(uint32_t n, uint64_t m, uint64_t seed, int nthreads, kcg_graph** out);
+/* Chung-Lu: weights ~ (i+1)^(-1/(gamma-1)) clipped to [wmin, wmax] and scaled to mean avg_deg;
+ * round(n*avg_deg/2) endpoint pairs drawn proportional to weight; ids permuted by the seed. */
+int kcg_gen_chung_lu(uint32_t n, double gamma, double wmin, double wmax, double avg_deg,
+                     uint64_t seed, int nthreads, kcg_graph** out);
+/* R-MAT: ndraws pairs on 2^scale ids with quadrant probabilities (a, b, c, 1-a-b-c);
+ * ids permuted by the seed (Graph500 convention). */
+int kcg_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c,
+                 uint64_t seed, int nthreads, kcg_graph** out);
+
+/* Write the pair list a generator would feed the loader as a 'u v' text edge list
+ * (one line per undirected edge u<v of the built CSR). */
+int kcg_write_edgelist(const kcg_graph* g, const char* path);
+
+/* splitmix64, exposed so the Python side derives per-graph seeds identically. */
+uint64_t kcg_splitmix64(uint64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KCORE_GRAPH_H */

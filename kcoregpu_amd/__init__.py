@@ -1,0 +1,268 @@
+"""kcoregpu_amd — k-core decomposition on AMD Instinct MI355X (gfx950).
+
+Python host-side mirror of the reference's interface (paths relative to the reference repo):
+
+* :class:`Graph` mirrors ``class Graph`` (inc/graph.h:7-23, src/graph.cpp): ``V``, ``E`` (arcs),
+  ``neighbors``, ``neighbors_offset``, ``degrees``, the text loader with the serialized
+  ``./output/serialized-<basename>`` cache, ``writeSerialized`` and ``writeKCoreToDisk``.
+* :func:`kcore` mirrors ``int kcore(Graph&)`` (src/ours.cc:117-160) but returns the core numbers
+  the reference leaves on the device, with the peel time and ``kmax``.
+* :class:`Engine` / :class:`DeviceGraph` expose the C ABI of ``libkcore_hip.so``
+  (include/kcore.h): a resident device graph peeled as often as wanted.
+
+Every compute call goes through the HIP library; nothing here computes core numbers itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4}
+FLAG_NO_LEVEL_SKIP = 1
+
+KCG_ERRORS = {1: "I/O error", 2: "vertex id or arc count beyond uint32", 3: "bad argument",
+              4: "malformed serialized cache"}
+
+
+class KCoreError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {status}")
+
+
+def _u32(a: np.ndarray):
+    return a.ctypes.data_as(N.u32p)
+
+
+class Graph:
+    """Host CSR graph with the reference loader's semantics (src/graph.cpp:58-124)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        L = N.host()
+        V = C.c_uint32()
+        arcs = C.c_uint64()
+        dmax = C.c_uint32()
+        L.kcg_graph_info(self._h, C.byref(V), C.byref(arcs), C.byref(dmax))
+        self.V = V.value
+        self.arcs = arcs.value
+        self.dmax = dmax.value
+        off = N.u32p()
+        nbr = N.u32p()
+        deg = N.u32p()
+        L.kcg_graph_arrays(self._h, C.byref(off), C.byref(nbr), C.byref(deg))
+        self.neighbors_offset = np.ctypeslib.as_array(off, shape=(self.V + 1,))
+        self.neighbors = (np.ctypeslib.as_array(nbr, shape=(self.arcs,)) if self.arcs
+                          else np.zeros(0, np.uint32))
+        self.degrees = np.ctypeslib.as_array(deg, shape=(self.V,))
+        self.kmax = 0
+
+    # the reference's E is the arc count (src/graph.cpp:114, printed by main.cu:55)
+    @property
+    def E(self) -> int:
+        return self.arcs
+
+    @property
+    def m(self) -> int:
+        return self.arcs // 2
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            N.host().kcg_graph_free(h)
+            self._h = C.c_void_p(0)
+
+    @staticmethod
+    def _make(fn, *args) -> "Graph":
+        out = C.c_void_p()
+        st = fn(*args, C.byref(out))
+        if st != 0:
+            raise KCoreError(st, KCG_ERRORS.get(st, "graph error"))
+        return Graph(out.value)
+
+    # ---- loaders -------------------------------------------------------------------
+    @classmethod
+    def from_edgelist(cls, path: str, nthreads: int = 0) -> "Graph":
+        return cls._make(N.host().kcg_load_edgelist, os.fsencode(path), nthreads)
+
+    @classmethod
+    def load(cls, path: str, output_dir: str = "./output/", nthreads: int = 0) -> "Graph":
+        """Reference constructor: serialized cache first, else parse and write the cache."""
+        L = N.host()
+        out = C.c_void_p()
+        hit = C.c_int(0)
+        st = L.kcg_load_cached(os.fsencode(path), os.fsencode(output_dir), nthreads, C.byref(out),
+                               C.byref(hit))
+        if st != 0:
+            raise KCoreError(st, KCG_ERRORS.get(st, "graph error"))
+        g = Graph(out.value)
+        g.from_cache = bool(hit.value)
+        return g
+
+    @classmethod
+    def read_serialized(cls, path: str) -> "Graph":
+        return cls._make(N.host().kcg_read_serialized, os.fsencode(path))
+
+    @classmethod
+    def from_pairs(cls, src, dst, nthreads: int = 0) -> "Graph":
+        s = np.ascontiguousarray(src, dtype=np.uint32)
+        t = np.ascontiguousarray(dst, dtype=np.uint32)
+        if s.shape != t.shape:
+            raise ValueError("src and dst differ in length")
+        return cls._make(N.host().kcg_from_pairs, _u32(s), _u32(t), s.size, nthreads)
+
+    @classmethod
+    def from_csr(cls, offsets, neighbors) -> "Graph":
+        o = np.ascontiguousarray(offsets, dtype=np.uint32)
+        n = np.ascontiguousarray(neighbors, dtype=np.uint32)
+        return cls._make(N.host().kcg_from_csr, o.size - 1, n.size, _u32(o), _u32(n))
+
+    # ---- generators ----------------------------------------------------------------
+    @classmethod
+    def erdos_renyi(cls, n: int, m: int, seed: int, nthreads: int = 0) -> "Graph":
+        return cls._make(N.host().kcg_gen_er, n, m, seed, nthreads)
+
+    @classmethod
+    def chung_lu(cls, n: int, seed: int, gamma: float = 2.1, wmin: float = 1.0,
+                 wmax: float = 2800.0, avg_deg: float = 12.0, nthreads: int = 0) -> "Graph":
+        return cls._make(N.host().kcg_gen_chung_lu, n, gamma, wmin, wmax, avg_deg, seed, nthreads)
+
+    @classmethod
+    def rmat(cls, scale: int, seed: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, ndraws: int | None = None, nthreads: int = 0) -> "Graph":
+        nd = ndraws if ndraws is not None else edge_factor << scale
+        return cls._make(N.host().kcg_gen_rmat, scale, nd, a, b, c, seed, nthreads)
+
+    # ---- writers (reference src/graph.cpp:35-54, :126-140) ---------------------------
+    def write_serialized(self, path: str) -> None:
+        st = N.host().kcg_write_serialized(self._h, os.fsencode(path))
+        if st:
+            raise KCoreError(st, "writeSerialized")
+
+    def write_edgelist(self, path: str) -> None:
+        st = N.host().kcg_write_edgelist(self._h, os.fsencode(path))
+        if st:
+            raise KCoreError(st, "write_edgelist")
+
+
+def write_kcore_json(path: str, core: np.ndarray) -> None:
+    c = np.ascontiguousarray(core, dtype=np.uint32)
+    st = N.host().kcg_write_kcore_json(os.fsencode(path), c.size, _u32(c))
+    if st:
+        raise KCoreError(st, "writeKCoreToDisk")
+
+
+def splitmix64(x: int) -> int:
+    return N.host().kcg_splitmix64(x & 0xFFFFFFFFFFFFFFFF)
+
+
+# ---- GPU engine ----------------------------------------------------------------------
+
+@dataclass
+class PeelResult:
+    core: np.ndarray | None
+    kmax: int
+    stats: dict
+
+
+def _check(st: int, what: str):
+    if st != 0:
+        msg = N.hip().kc_status_str(st).decode()
+        raise KCoreError(st, f"{what}: {msg}")
+
+
+class Engine:
+    """One HIP device context (kc_open). Not shared across threads."""
+
+    def __init__(self, device: int = 0):
+        self._c = C.c_void_p()
+        _check(N.hip().kc_open(device, C.byref(self._c)), "kc_open")
+
+    def close(self):
+        if self._c.value:
+            N.hip().kc_close(self._c)
+            self._c = C.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def selftest(self) -> None:
+        _check(N.hip().kc_selftest(self._c), "kc_selftest")
+
+    def upload(self, g: Graph) -> "DeviceGraph":
+        h = N.kc_csr(g.V, g.arcs, _u32(g.neighbors_offset),
+                     _u32(g.neighbors) if g.arcs else N.u32p(), _u32(g.degrees))
+        out = C.c_void_p()
+        _check(N.hip().kc_graph_upload(self._c, C.byref(h), C.byref(out)), "kc_graph_upload")
+        return DeviceGraph(self, out, g.V)
+
+
+class DeviceGraph:
+    def __init__(self, eng: Engine, handle: C.c_void_p, V: int):
+        self.eng = eng
+        self._g = handle
+        self.V = V
+        self.kmax = None
+
+    def free(self):
+        if self._g.value:
+            N.hip().kc_graph_free(self._g)
+            self._g = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def peel(self, variant: str = "auto", block: int = 0, lds_depth: int = 0,
+             heavy_threshold: int = 0, flags: int = 0, download: bool = True) -> PeelResult:
+        p = N.kc_params(VARIANTS[variant], block, lds_depth, heavy_threshold, flags)
+        st = N.kc_stats()
+        kmax = C.c_uint32()
+        core = np.empty(self.V, np.uint32) if download else None
+        _check(N.hip().kc_peel(self.eng._c, self._g, C.byref(p),
+                               _u32(core) if download else N.u32p(), C.byref(kmax),
+                               C.byref(st)), "kc_peel")
+        self.kmax = kmax.value
+        return PeelResult(core, kmax.value, st.as_dict())
+
+    def core(self) -> np.ndarray:
+        out = np.empty(self.V, np.uint32)
+        _check(N.hip().kc_graph_download_core(self.eng._c, self._g, _u32(out)), "download_core")
+        return out
+
+    def order(self) -> tuple[np.ndarray, np.ndarray]:
+        if self.kmax is None:
+            raise RuntimeError("peel first")
+        order = np.empty(self.V, np.uint32)
+        shells = np.empty(self.kmax + 2, np.uint32)
+        _check(N.hip().kc_graph_download_order(self.eng._c, self._g, _u32(order), _u32(shells)),
+               "download_order")
+        return order, shells
+
+
+def kcore(g: Graph, variant: str = "auto", engine: Engine | None = None) -> PeelResult:
+    """``int kcore(Graph&)`` (src/ours.cc:117-160), returning core numbers, kmax and timings."""
+    eng = engine or Engine()
+    dg = eng.upload(g)
+    try:
+        r = dg.peel(variant)
+    finally:
+        dg.free()
+    g.kmax = r.kmax
+    return r

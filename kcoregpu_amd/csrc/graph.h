@@ -1,0 +1,89 @@
+// graph.h — host graph layer: reference-compatible `Graph` class plus the CSR builder,
+// serialized cache and writers behind the C ABI in include/kcore_graph.h.
+//
+// `class Graph` keeps the public fields and member functions of the reference's
+// inc/graph.h:7-23 so host code written against that class compiles unchanged; its
+// behaviour (console lines, cache path ./output/serialized-<basename>, JSON writer)
+// follows src/graph.cpp:9-161.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace kcg {
+
+// A CSR graph as the loader builds it: V vertices, `arcs` = 2 * undirected edges,
+// adjacency ascending and duplicate-free, no self loops (reference src/graph.cpp:97-122).
+struct Csr {
+    uint32_t V = 0;
+    uint64_t arcs = 0;
+    uint32_t dmax = 0;
+    std::vector<uint32_t> offsets;    // V + 1
+    std::vector<uint32_t> neighbors;  // arcs
+    std::vector<uint32_t> degrees;    // V
+};
+
+// Status codes shared with the C ABI.
+enum Status { OK = 0, ERR_IO = 1, ERR_RANGE = 2, ERR_ARG = 3, ERR_FORMAT = 4 };
+
+// Build a CSR from a pair list with the loader's rules: pairs with s == t are dropped,
+// V = 1 + max id over the remaining pairs (1 when none remain), every pair is inserted in
+// both directions, duplicates removed, adjacency sorted ascending.
+int build_csr(const uint32_t* s, const uint32_t* t, uint64_t npairs, int nthreads, Csr& out);
+
+// Parse a text edge list with the reference's per-line rules (src/graph.cpp:68-96):
+// lines whose first char is alphabetic are skipped, each other line is read as
+// `iss >> s >> t` (values kept from the previous line when extraction fails), self loops
+// are dropped. Appends the kept pairs.
+int parse_edgelist(const std::string& path, int nthreads, std::vector<uint32_t>& s,
+                   std::vector<uint32_t>& t);
+
+int load_edgelist(const std::string& path, int nthreads, Csr& out);
+int read_serialized(const std::string& path, Csr& out);
+int write_serialized(const Csr& g, const std::string& path);
+int write_kcore_json(const std::string& path, uint32_t V, const uint32_t* core);
+int write_core_u32(const std::string& path, uint32_t V, const uint32_t* core);
+int write_edgelist(const Csr& g, const std::string& path);
+
+std::string basename_of(const std::string& path);
+
+// ---- generators (generators.cpp) ----
+uint64_t splitmix64(uint64_t x);
+int gen_er(uint32_t n, uint64_t m, uint64_t seed, int nthreads, Csr& out);
+int gen_chung_lu(uint32_t n, double gamma, double wmin, double wmax, double avg_deg, uint64_t seed,
+                 int nthreads, Csr& out);
+int gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c, uint64_t seed,
+             int nthreads, Csr& out);
+
+}  // namespace kcg
+
+// Reference-compatible graph object (inc/graph.h:7-23). Arrays are owned by the object.
+class Graph {
+public:
+    unsigned int V = 0;
+    unsigned int E = 0;  // arcs = 2 * undirected edges, as printed by the reference driver
+    unsigned int AVG_DEGREE = 0;
+    unsigned int* neighbors = nullptr;
+    unsigned int* neighbors_offset = nullptr;
+    unsigned int* degrees = nullptr;
+    unsigned int kmax = 0, dmax = 0;
+
+    explicit Graph(std::string input_file);
+    Graph();
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    bool readSerialized(std::string input_file);
+    void writeSerialized(std::string input_file);
+    // Reference readFile exits the process on a missing file; this one returns false after
+    // printing the same line so the driver decides.
+    bool readFile(std::string input_file);
+    void writeKCoreToDisk(std::string file);
+    ~Graph();
+
+    // Directory holding the serialized cache and outputs (reference OUTPUT_LOC "./output/").
+    static std::string output_dir;
+
+private:
+    kcg::Csr csr_;
+    void adopt();
+};

@@ -1,0 +1,133 @@
+"""Loader, serialized cache, JSON writer and generators against the reference's own loader
+(fixtures written by tests/golden/make_golden.py with oracle/_ref/ref_loader, built from the
+reference's src/graph.cpp) and against properties of the generators."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import kcoregpu_amd as K
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "loader")
+CASES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".txt"))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_loader_matches_reference_fixture(case, tmp_path):
+    src = os.path.join(GOLDEN, case + ".txt")
+    exp = os.path.join(GOLDEN, case + ".expected")
+    z = np.load(os.path.join(exp, "csr.npz"))
+    g = K.Graph.from_edgelist(src)
+    assert g.V == int(z["V"]) and g.E == int(z["E"])
+    assert (g.neighbors_offset == z["off"]).all()
+    assert (g.neighbors == z["nbr"]).all()
+    assert (g.degrees == z["deg"]).all()
+    # serialized cache, byte for byte
+    out = tmp_path / "ser"
+    g.write_serialized(str(out))
+    assert out.read_bytes() == open(os.path.join(exp, "serialized"), "rb").read()
+    # the cache reads back to the same CSR
+    h = K.Graph.read_serialized(str(out))
+    assert h.V == g.V and (h.neighbors == g.neighbors).all() and (h.degrees == g.degrees).all()
+    # JSON core writer, byte for byte
+    j = tmp_path / "core.json"
+    K.write_kcore_json(str(j), z["core"])
+    assert j.read_bytes() == open(os.path.join(exp, "pkc-kcore.json"), "rb").read()
+
+
+def test_load_cached_uses_basename_cache(tmp_path):
+    src = tmp_path / "g.txt"
+    src.write_text("0 1\n1 2\n")
+    outdir = tmp_path / "output"
+    outdir.mkdir()
+    g = K.Graph.load(str(src), str(outdir))
+    assert not g.from_cache and (outdir / "serialized-g.txt").exists()
+    # the reference keys the cache on the basename only: a different file with the same name
+    # in another directory hits the same cache (src/graph.cpp:10,36)
+    other = tmp_path / "sub"
+    other.mkdir()
+    (other / "g.txt").write_text("5 6\n")
+    g2 = K.Graph.load(str(other / "g.txt"), str(outdir))
+    assert g2.from_cache and g2.V == 3
+
+
+def test_missing_file_is_an_error(tmp_path):
+    with pytest.raises(K.KCoreError):
+        K.Graph.from_edgelist(str(tmp_path / "nope.txt"))
+
+
+def test_json_writer_no_nonzero_core(tmp_path):
+    j = tmp_path / "c.json"
+    K.write_kcore_json(str(j), np.zeros(4, np.uint32))
+    assert j.read_bytes() == b"{ "
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_LOADER), reason="reference loader not built here")
+@pytest.mark.parametrize("gen", ["er", "cl", "rmat"])
+def test_generated_text_through_reference_loader(gen, tmp_path):
+    g = {"er": lambda: K.Graph.erdos_renyi(3000, 20000, 5),
+         "cl": lambda: K.Graph.chung_lu(4000, 6),
+         "rmat": lambda: K.Graph.rmat(12, 7)}[gen]()
+    txt = tmp_path / f"{gen}.txt"
+    g.write_edgelist(str(txt))
+    work = tmp_path / "w"
+    (work / "output").mkdir(parents=True)
+    pre = str(work / "dump")
+    subprocess.run([O.REF_LOADER, str(txt), str(work), pre], check=True, stdout=subprocess.DEVNULL)
+    off = np.fromfile(pre + ".off", np.uint32)
+    nbr = np.fromfile(pre + ".nbr", np.uint32)
+    mine = K.Graph.from_edgelist(str(txt))
+    # the generated edge list's max id can be below the generator's n: compare what loads
+    assert (mine.neighbors_offset == off).all() and (mine.neighbors == nbr).all()
+    ser = open(work / "output" / f"serialized-{gen}.txt", "rb").read()
+    mine.write_serialized(str(tmp_path / "ser"))
+    assert (tmp_path / "ser").read_bytes() == ser
+
+
+def test_generators_deterministic_across_threads():
+    for mk in (lambda nt: K.Graph.erdos_renyi(5000, 40000, 3, nthreads=nt),
+               lambda nt: K.Graph.chung_lu(8000, 4, nthreads=nt),
+               lambda nt: K.Graph.rmat(13, 9, nthreads=nt)):
+        a, b = mk(1), mk(7)
+        assert a.V == b.V and (a.neighbors_offset == b.neighbors_offset).all()
+        assert (a.neighbors == b.neighbors).all()
+
+
+def test_generator_shapes():
+    g = K.Graph.erdos_renyi(100000, 1000000, 1)
+    assert g.arcs == 2_000_000  # exactly m distinct undirected edges
+    assert g.V <= 100000
+    off, nbr = g.neighbors_offset, g.neighbors
+    # symmetric, sorted, loop-free
+    src = np.repeat(np.arange(g.V, dtype=np.uint32), np.diff(off))
+    assert (src != nbr).all()
+    fwd = np.sort(src.astype(np.uint64) << 32 | nbr)
+    bwd = np.sort(nbr.astype(np.uint64) << 32 | src)
+    assert (fwd == bwd).all()
+    assert all((np.diff(nbr[off[v]:off[v + 1]].astype(np.int64)) > 0).all() for v in range(0, g.V, 997))
+    cl = K.Graph.chung_lu(400000, 2)
+    assert 2_000_000 < cl.m < 2_500_000 and cl.dmax <= 4000
+    r = K.Graph.rmat(16, 3)
+    assert r.V <= 1 << 16 and r.dmax > 1000
+
+
+def test_text_roundtrip(tmp_path):
+    g = K.Graph.rmat(12, 2)
+    p = tmp_path / "g.txt"
+    g.write_edgelist(str(p))
+    h = K.Graph.from_edgelist(str(p))
+    # ids beyond the last non-isolated vertex disappear in text form (V = 1 + max id)
+    assert h.V <= g.V
+    assert (h.neighbors_offset == g.neighbors_offset[:h.V + 1]).all()
+    assert (h.neighbors == g.neighbors).all()
+
+
+def test_pairs_rules():
+    g = K.Graph.from_pairs([0, 1, 1, 3, 2], [1, 0, 1, 3, 4])
+    assert g.V == 5  # self loops do not count towards V: max non-loop id is 4
+    assert list(g.degrees) == [1, 1, 1, 0, 1]
+    e = K.Graph.from_pairs([7], [7])
+    assert e.V == 1 and e.arcs == 0
