@@ -77,10 +77,12 @@ typedef struct {
     uint32_t heavy_threshold; /* degree above which a vertex's arcs are split into chunks shared
                                  by many waves; 0 = default, UINT32_MAX = never split */
     uint32_t flags;           /* KC_FLAG_* */
+    uint32_t chunk_arcs;      /* arcs per chunk; 0 = default */
 } kc_params;
 
 enum {
-    KC_FLAG_NO_LEVEL_SKIP = 1u << 0 /* scan every level 0..kmax like the reference, even empty ones */
+    KC_FLAG_NO_LEVEL_SKIP = 1u << 0, /* scan every level 0..kmax like the reference, even empty ones */
+    KC_FLAG_TRACE = 1u << 8          /* record a device timestamp at every grid barrier */
 };
 
 typedef struct {
@@ -90,12 +92,14 @@ typedef struct {
     double ms_kernel;         /* device time of the peeling kernel(s) alone, without state reset */
     uint32_t kmax;
     uint32_t levels;          /* levels whose degree==level scan ran */
+    uint32_t subrounds;       /* processing sub-rounds (grid-synchronous steps) over all levels */
     uint64_t scanned_entries; /* degree entries read by the select scans */
     uint64_t bytes_alg;       /* algorithmic bytes: 4*scanned + 16*n + 20*m */
     uint64_t bytes_scan;      /* 4*scanned_entries */
     uint64_t bytes_edge;      /* 16*n + 20*m */
     uint32_t grid;            /* workgroups launched */
     uint32_t block;           /* threads per workgroup */
+    uint32_t participants;    /* workgroups resident at start that ran the peel (<= grid) */
 } kc_stats;
 
 const char* kc_status_str(kc_status s);
@@ -122,9 +126,21 @@ kc_status kc_graph_download_order(kc_ctx* ctx, const kc_graph* g, uint32_t* orde
 /* Device pointer of the core array (valid until kc_graph_free), for zero-copy consumers. */
 const uint32_t* kc_graph_core_device(const kc_graph* g);
 
-/* Diagnostics (no reference counterpart): checks the wave64 DPP prefix sum the kernels use
- * against a host prefix sum on one wavefront. KC_OK when they agree. */
+/* Diagnostics (no reference counterpart). kc_selftest checks the wave64 DPP prefix sum the
+ * kernels use against a host prefix sum on one wavefront (KC_OK when they agree).
+ * kc_graph_download_trace returns the barrier timestamps of the last peel run with
+ * KC_FLAG_TRACE: pairs (kind << 56 | subround << 32 | level, wall-clock ticks), kind 1 = after
+ * select, 2 = after a sub-round, 3 = level end; ticks run at kc_wall_clock_khz kHz. */
 kc_status kc_selftest(kc_ctx* ctx);
+kc_status kc_graph_download_trace(kc_ctx* ctx, const kc_graph* g, uint64_t* rec, uint32_t cap,
+                                  uint32_t* n_out);
+uint64_t kc_wall_clock_khz(const kc_ctx* ctx);
+/* Per-workgroup loop positions (8 words each) and the control block of the last peel, for
+ * diagnosing a KC_ERR_DEVICE time-out. */
+kc_status kc_graph_download_debug(kc_ctx* ctx, const kc_graph* g, uint32_t* words, uint32_t cap,
+                                  uint32_t* n_out, uint32_t* ctrl_out, uint32_t ctrl_cap);
+/* Workgroups of the peel kernel the runtime reports resident per CU, and the CU count. */
+kc_status kc_occupancy(kc_ctx* ctx, uint32_t block, int variant, uint32_t* per_cu, uint32_t* cus);
 
 #ifdef __cplusplus
 }

@@ -14,8 +14,10 @@ Every compute call goes through the HIP library; nothing here computes core numb
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -24,6 +26,7 @@ from . import _native as N
 
 VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4}
 FLAG_NO_LEVEL_SKIP = 1
+FLAG_TRACE = 1 << 8
 
 KCG_ERRORS = {1: "I/O error", 2: "vertex id or arc count beyond uint32", 3: "bad argument",
               4: "malformed serialized cache"}
@@ -176,14 +179,29 @@ def _check(st: int, what: str):
         raise KCoreError(st, f"{what}: {msg}")
 
 
+_ENGINES = weakref.WeakSet()
+
+
+@atexit.register
+def _close_engines():
+    # release device state before the HIP runtime's own exit handlers run
+    for e in list(_ENGINES):
+        e.close()
+
+
 class Engine:
     """One HIP device context (kc_open). Not shared across threads."""
 
     def __init__(self, device: int = 0):
         self._c = C.c_void_p()
+        self._graphs = weakref.WeakSet()
         _check(N.hip().kc_open(device, C.byref(self._c)), "kc_open")
+        _ENGINES.add(self)
 
     def close(self):
+        # device graphs hold the context's stream: release them first
+        for g in list(self._graphs):
+            g.free()
         if self._c.value:
             N.hip().kc_close(self._c)
             self._c = C.c_void_p(0)
@@ -208,7 +226,9 @@ class Engine:
                      _u32(g.neighbors) if g.arcs else N.u32p(), _u32(g.degrees))
         out = C.c_void_p()
         _check(N.hip().kc_graph_upload(self._c, C.byref(h), C.byref(out)), "kc_graph_upload")
-        return DeviceGraph(self, out, g.V)
+        dg = DeviceGraph(self, out, g.V)
+        self._graphs.add(dg)
+        return dg
 
 
 class DeviceGraph:
@@ -230,8 +250,9 @@ class DeviceGraph:
             pass
 
     def peel(self, variant: str = "auto", block: int = 0, lds_depth: int = 0,
-             heavy_threshold: int = 0, flags: int = 0, download: bool = True) -> PeelResult:
-        p = N.kc_params(VARIANTS[variant], block, lds_depth, heavy_threshold, flags)
+             heavy_threshold: int = 0, flags: int = 0, download: bool = True,
+             chunk_arcs: int = 0) -> PeelResult:
+        p = N.kc_params(VARIANTS[variant], block, lds_depth, heavy_threshold, flags, chunk_arcs)
         st = N.kc_stats()
         kmax = C.c_uint32()
         core = np.empty(self.V, np.uint32) if download else None
@@ -244,6 +265,22 @@ class DeviceGraph:
     def core(self) -> np.ndarray:
         out = np.empty(self.V, np.uint32)
         _check(N.hip().kc_graph_download_core(self.eng._c, self._g, _u32(out)), "download_core")
+        return out
+
+    def trace(self) -> list[tuple[str, int, int, float]]:
+        """Barrier timestamps of the last peel run with FLAG_TRACE: (kind, level, subround, us)."""
+        cap = 1 << 16
+        rec = np.zeros(2 * cap, np.uint64)
+        n = C.c_uint32()
+        _check(N.hip().kc_graph_download_trace(self.eng._c, self._g, rec.ctypes.data_as(N.u64p),
+                                                cap, C.byref(n)), "download_trace")
+        khz = N.hip().kc_wall_clock_khz(self.eng._c)
+        kinds = {1: "scan", 2: "sub", 3: "level"}
+        out = []
+        for i in range(n.value):
+            w, t = int(rec[2 * i]), int(rec[2 * i + 1])
+            out.append((kinds.get(w >> 56, "?"), w & 0xFFFFFFFF, (w >> 32) & 0xFFFFFF, t * 1e3 / khz))
+        out.sort(key=lambda r: r[3])
         return out
 
     def order(self) -> tuple[np.ndarray, np.ndarray]:

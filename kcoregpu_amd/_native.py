@@ -41,15 +41,16 @@ class kc_csr(C.Structure):
 
 class kc_params(C.Structure):
     _fields_ = [("variant", C.c_int), ("block", C.c_uint32), ("lds_depth", C.c_uint32),
-                ("heavy_threshold", C.c_uint32), ("flags", C.c_uint32)]
+                ("heavy_threshold", C.c_uint32), ("flags", C.c_uint32), ("chunk_arcs", C.c_uint32)]
 
 
 class kc_stats(C.Structure):
     _fields_ = [("ms_peel", C.c_double), ("ms_h2d", C.c_double), ("ms_d2h", C.c_double),
                 ("ms_kernel", C.c_double),
-                ("kmax", C.c_uint32), ("levels", C.c_uint32), ("scanned_entries", C.c_uint64),
+                ("kmax", C.c_uint32), ("levels", C.c_uint32), ("subrounds", C.c_uint32),
+                ("scanned_entries", C.c_uint64),
                 ("bytes_alg", C.c_uint64), ("bytes_scan", C.c_uint64), ("bytes_edge", C.c_uint64),
-                ("grid", C.c_uint32), ("block", C.c_uint32)]
+                ("grid", C.c_uint32), ("block", C.c_uint32), ("participants", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -109,5 +110,10 @@ def hip() -> C.CDLL:
         L.kc_graph_core_device.argtypes = [vp]
         L.kc_graph_core_device.restype = vp
         L.kc_selftest.argtypes = [vp]
+        L.kc_graph_download_trace.argtypes = [vp, vp, u64p, C.c_uint32, u32p]
+        L.kc_wall_clock_khz.argtypes = [vp]
+        L.kc_wall_clock_khz.restype = C.c_uint64
+        L.kc_occupancy.argtypes = [vp, C.c_uint32, C.c_int, u32p, u32p]
+        L.kc_graph_download_debug.argtypes = [vp, vp, u32p, C.c_uint32, u32p, u32p, C.c_uint32]
         _hip = L
     return _hip

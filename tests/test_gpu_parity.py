@@ -100,7 +100,8 @@ def test_chung_lu_400k_variants(engine):
     core, kmax = O.bz(g.neighbors_offset, g.neighbors)
     check_against(g, peel_all(engine, g), core, kmax)
     # hub arcs cut into chunks at the smallest threshold, and never cut
-    check_against(g, peel_all(engine, g, ["wave"], heavy_threshold=256), core, kmax)
+    check_against(g, peel_all(engine, g, ["wave"], heavy_threshold=16, chunk_arcs=64), core, kmax)
+    check_against(g, peel_all(engine, g, ["lds"], heavy_threshold=256, chunk_arcs=1000), core, kmax)
     check_against(g, peel_all(engine, g, ["wave"], heavy_threshold=0xFFFFFFFF), core, kmax)
 
 
@@ -114,8 +115,9 @@ def test_star_and_hub_chunks(engine):
     g = K.Graph.from_pairs(np.array(src), np.array(dst))
     core, kmax = O.bz(g.neighbors_offset, g.neighbors)
     assert kmax == 9
-    for h in (256, 1000, 4096):
-        check_against(g, peel_all(engine, g, ["wave"], heavy_threshold=h), core, kmax)
+    for h, c in ((16, 64), (64, 512), (1000, 4096), (0xFFFFFFFF, 0)):
+        check_against(g, peel_all(engine, g, ["wave", "prefetch"], heavy_threshold=h, chunk_arcs=c),
+                      core, kmax)
 
 
 def test_degenerate_inputs(engine):
