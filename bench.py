@@ -26,6 +26,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBPS = 8000.0
+PROFILE = os.path.join(HERE, "profiles", "peel_persistent_rmat24.md")
+
+
+def profiled_traffic(args):
+    """HBM bytes per launch of the peel kernel, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this same default workload (None for any other workload)."""
+    if (args.scale, args.edge_factor, args.seed, args.variant, args.block, args.heavy,
+            args.flags) != (24, 16, 3, "auto", 0, 0, 0) or not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def log(*a):
@@ -138,7 +151,7 @@ def main():
                    "levels_scanned": st["levels"], "variant": args.variant,
                    "grid": st["grid"], "block": st["block"], "flags": args.flags},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": profiled_traffic(args),
                      "kernel": "peel_persistent", "kernel_ms": float(np.mean(kernel_ms)),
                      "bytes_alg": st["bytes_alg"], "bytes_scan": st["bytes_scan"],
                      "bytes_edge": st["bytes_edge"]},

@@ -101,9 +101,10 @@ static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out
     *grid_out = grid;
     PeelArgs a = args;
     if (a.lq_depth > (uint32_t)WgState<BLOCK>::LQ) a.lq_depth = WgState<BLOCK>::LQ;
-    void* kargs[] = {&a};
-    KC_TRY(hipLaunchCooperativeKernel(reinterpret_cast<void*>(peel_persistent<BLOCK, PREFETCH>),
-                                      dim3(grid), dim3(BLOCK), kargs, 0, c->stream));
+    // A plain launch: the kernel's elastic start (register_workgroup) only lets workgroups
+    // that are resident together take part, so it needs no co-residency guarantee.
+    peel_persistent<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    KC_TRY(hipGetLastError());
     return KC_OK;
 }
 
