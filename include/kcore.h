@@ -126,6 +126,35 @@ kc_status kc_graph_download_order(kc_ctx* ctx, const kc_graph* g, uint32_t* orde
 /* Device pointer of the core array (valid until kc_graph_free), for zero-copy consumers. */
 const uint32_t* kc_graph_core_device(const kc_graph* g);
 
+/* ---- batched peeling (no reference counterpart) ----
+ * Many independent graphs packed into one CSR: graph g owns vertices [vtx_off[g], vtx_off[g+1])
+ * of the packed vertex arrays (local ids 0..V_g-1), arcs [arc_off[g], arc_off[g+1]) of the packed
+ * neighbour array (local ids), and V_g+1 local offsets starting at offsets[vtx_off[g] + g]
+ * (offsets relative to arc_off[g]). One persistent kernel peels all of them, one workgroup per
+ * graph at a time, largest graphs first, each graph with its own level counter. */
+typedef struct {
+    uint32_t G;
+    uint64_t total_vertices;     /* vtx_off[G] */
+    uint64_t total_arcs;         /* arc_off[G] */
+    const uint64_t* vtx_off;     /* G+1 */
+    const uint64_t* arc_off;     /* G+1 */
+    const uint32_t* offsets;     /* total_vertices + G */
+    const uint32_t* neighbors;   /* total_arcs */
+    const uint32_t* degrees;     /* total_vertices; NULL = from offsets */
+} kc_batch_csr;
+typedef struct kc_batch kc_batch;
+
+/* Copies the batch to the device and checks it there (offsets monotone, local ids < V_g). */
+kc_status kc_batch_upload(kc_ctx* ctx, const kc_batch_csr* b, kc_batch** out);
+void kc_batch_free(kc_batch* b);
+/* core_out: total_vertices entries (packed like the input), kmax_out: G entries; both may be
+ * NULL. stats: levels / subrounds summed over graphs, bytes_alg summed over graphs. Only
+ * kc_params.block and .flags (KC_FLAG_NO_LEVEL_SKIP) apply. */
+kc_status kc_batch_peel(kc_ctx* ctx, kc_batch* b, const kc_params* p, uint32_t* core_out,
+                        uint32_t* kmax_out, kc_stats* stats);
+/* Per-graph peel order, packed like the vertex arrays (local ids). */
+kc_status kc_batch_download_order(kc_ctx* ctx, const kc_batch* b, uint32_t* order_out);
+
 /* Diagnostics (no reference counterpart). kc_selftest checks the wave64 DPP prefix sum the
  * kernels use against a host prefix sum on one wavefront (KC_OK when they agree).
  * kc_graph_download_trace returns the barrier timestamps of the last peel run with

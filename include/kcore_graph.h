@@ -84,6 +84,26 @@ int kcg_write_edgelist(const kcg_graph* g, const char* path);
 /* splitmix64, exposed so the Python side derives per-graph seeds identically. */
 uint64_t kcg_splitmix64(uint64_t x);
 
+/* ---- batches: many graphs packed into one CSR (the layout of kc_batch_csr in kcore.h) ----
+ * Graph g owns vertices [vtx_off[g], vtx_off[g+1]) of the packed vertex arrays (local ids
+ * 0..V_g-1), arcs [arc_off[g], arc_off[g+1]) of the packed neighbour array (local ids) and the
+ * V_g+1 local offsets starting at offsets[vtx_off[g] + g]. */
+typedef struct kcg_batch kcg_batch;
+
+/* G graphs; graph i has n_i = round(exp(U(ln nmin, ln nmax))) vertices and mean degree
+ * U(dmin, dmax); even i are Erdos-Renyi G(n, round(n*avg/2)), odd i Chung-Lu with gamma
+ * (weights clipped to [1, min(2800, n/2)]); every draw derives from (seed, i). */
+int kcg_gen_batch(uint32_t G, uint32_t nmin, uint32_t nmax, double dmin, double dmax, double gamma,
+                  uint64_t seed, int nthreads, kcg_batch** out);
+/* Pack existing graphs (copied). */
+int kcg_batch_from_graphs(const kcg_graph* const* graphs, uint32_t G, kcg_batch** out);
+void kcg_batch_free(kcg_batch* b);
+void kcg_batch_info(const kcg_batch* b, uint32_t* G, uint64_t* total_vertices, uint64_t* total_arcs);
+void kcg_batch_arrays(const kcg_batch* b, const uint64_t** vtx_off, const uint64_t** arc_off,
+                      const uint32_t** offsets, const uint32_t** neighbors, const uint32_t** degrees);
+/* Copy graph g out of the batch as a stand-alone graph. */
+int kcg_batch_graph(const kcg_batch* b, uint32_t g, kcg_graph** out);
+
 #ifdef __cplusplus
 }
 #endif

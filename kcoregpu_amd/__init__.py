@@ -153,6 +153,58 @@ class Graph:
             raise KCoreError(st, "write_edgelist")
 
 
+class Batch:
+    """Many graphs packed into one CSR (layout of kc_batch_csr, include/kcore.h)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        L = N.host()
+        G = C.c_uint32()
+        nv = C.c_uint64()
+        na = C.c_uint64()
+        L.kcg_batch_info(self._h, C.byref(G), C.byref(nv), C.byref(na))
+        self.G, self.total_vertices, self.total_arcs = G.value, nv.value, na.value
+        vo, ao, of, nb, dg = N.u64p(), N.u64p(), N.u32p(), N.u32p(), N.u32p()
+        L.kcg_batch_arrays(self._h, C.byref(vo), C.byref(ao), C.byref(of), C.byref(nb), C.byref(dg))
+        self.vtx_off = np.ctypeslib.as_array(vo, shape=(self.G + 1,))
+        self.arc_off = np.ctypeslib.as_array(ao, shape=(self.G + 1,))
+        self.offsets = np.ctypeslib.as_array(of, shape=(self.total_vertices + self.G,))
+        self.neighbors = (np.ctypeslib.as_array(nb, shape=(self.total_arcs,)) if self.total_arcs
+                          else np.zeros(0, np.uint32))
+        self.degrees = (np.ctypeslib.as_array(dg, shape=(self.total_vertices,)) if self.total_vertices
+                        else np.zeros(0, np.uint32))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            N.host().kcg_batch_free(h)
+            self._h = C.c_void_p(0)
+
+    @classmethod
+    def generate(cls, G: int, seed: int, nmin: int = 1000, nmax: int = 200000, dmin: float = 4.0,
+                 dmax: float = 32.0, gamma: float = 2.1, nthreads: int = 0) -> "Batch":
+        out = C.c_void_p()
+        st = N.host().kcg_gen_batch(G, nmin, nmax, dmin, dmax, gamma, seed, nthreads, C.byref(out))
+        if st:
+            raise KCoreError(st, "kcg_gen_batch")
+        return cls(out.value)
+
+    @classmethod
+    def from_graphs(cls, graphs: list) -> "Batch":
+        arr = (C.c_void_p * len(graphs))(*[g._h.value for g in graphs])
+        out = C.c_void_p()
+        st = N.host().kcg_batch_from_graphs(arr, len(graphs), C.byref(out))
+        if st:
+            raise KCoreError(st, "kcg_batch_from_graphs")
+        return cls(out.value)
+
+    def graph(self, g: int) -> Graph:
+        return Graph._make(N.host().kcg_batch_graph, self._h, g)
+
+    def vertices(self, g: int) -> slice:
+        return slice(int(self.vtx_off[g]), int(self.vtx_off[g + 1]))
+
+
 def write_kcore_json(path: str, core: np.ndarray) -> None:
     c = np.ascontiguousarray(core, dtype=np.uint32)
     st = N.host().kcg_write_kcore_json(os.fsencode(path), c.size, _u32(c))
@@ -229,6 +281,51 @@ class Engine:
         dg = DeviceGraph(self, out, g.V)
         self._graphs.add(dg)
         return dg
+
+    def upload_batch(self, b: Batch) -> DeviceBatch:
+        h = N.kc_batch_csr(b.G, b.total_vertices, b.total_arcs, b.vtx_off.ctypes.data_as(N.u64p),
+                           b.arc_off.ctypes.data_as(N.u64p), _u32(b.offsets),
+                           _u32(b.neighbors) if b.total_arcs else N.u32p(),
+                           _u32(b.degrees) if b.total_vertices else N.u32p())
+        out = C.c_void_p()
+        _check(N.hip().kc_batch_upload(self._c, C.byref(h), C.byref(out)), "kc_batch_upload")
+        db = DeviceBatch(self, out, b)
+        self._graphs.add(db)
+        return db
+
+
+class DeviceBatch:
+    def __init__(self, eng: "Engine", handle: C.c_void_p, batch: Batch):
+        self.eng = eng
+        self._b = handle
+        self.G = batch.G
+        self.total_vertices = batch.total_vertices
+
+    def free(self):
+        if self._b.value:
+            N.hip().kc_batch_free(self._b)
+            self._b = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def peel(self, block: int = 0, flags: int = 0, download: bool = True):
+        p = N.kc_params(0, block, 0, 0, flags, 0)
+        st = N.kc_stats()
+        core = np.empty(self.total_vertices, np.uint32) if download else None
+        kmax = np.empty(self.G, np.uint32)
+        _check(N.hip().kc_batch_peel(self.eng._c, self._b, C.byref(p),
+                                     _u32(core) if download else N.u32p(), _u32(kmax),
+                                     C.byref(st)), "kc_batch_peel")
+        return core, kmax, st.as_dict()
+
+    def order(self) -> np.ndarray:
+        out = np.empty(self.total_vertices, np.uint32)
+        _check(N.hip().kc_batch_download_order(self.eng._c, self._b, _u32(out)), "batch order")
+        return out
 
 
 class DeviceGraph:

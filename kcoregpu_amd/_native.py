@@ -39,6 +39,12 @@ class kc_csr(C.Structure):
                 ("neighbors", u32p), ("degrees", u32p)]
 
 
+class kc_batch_csr(C.Structure):
+    _fields_ = [("G", C.c_uint32), ("total_vertices", C.c_uint64), ("total_arcs", C.c_uint64),
+                ("vtx_off", u64p), ("arc_off", u64p), ("offsets", u32p), ("neighbors", u32p),
+                ("degrees", u32p)]
+
+
 class kc_params(C.Structure):
     _fields_ = [("variant", C.c_int), ("block", C.c_uint32), ("lds_depth", C.c_uint32),
                 ("heavy_threshold", C.c_uint32), ("flags", C.c_uint32), ("chunk_arcs", C.c_uint32)]
@@ -84,6 +90,17 @@ def host() -> C.CDLL:
         L.kcg_write_edgelist.argtypes = [vp, C.c_char_p]
         L.kcg_splitmix64.argtypes = [C.c_uint64]
         L.kcg_splitmix64.restype = C.c_uint64
+        L.kcg_gen_batch.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                    C.c_double, C.c_uint64, C.c_int, pp]
+        L.kcg_batch_from_graphs.argtypes = [C.POINTER(vp), C.c_uint32, pp]
+        L.kcg_batch_free.argtypes = [vp]
+        L.kcg_batch_free.restype = None
+        L.kcg_batch_info.argtypes = [vp, u32p, u64p, u64p]
+        L.kcg_batch_info.restype = None
+        L.kcg_batch_arrays.argtypes = [vp, C.POINTER(u64p), C.POINTER(u64p), C.POINTER(u32p),
+                                       C.POINTER(u32p), C.POINTER(u32p)]
+        L.kcg_batch_arrays.restype = None
+        L.kcg_batch_graph.argtypes = [vp, C.c_uint32, pp]
         _host = L
     return _host
 
@@ -114,6 +131,11 @@ def hip() -> C.CDLL:
         L.kc_wall_clock_khz.argtypes = [vp]
         L.kc_wall_clock_khz.restype = C.c_uint64
         L.kc_occupancy.argtypes = [vp, C.c_uint32, C.c_int, u32p, u32p]
+        L.kc_batch_upload.argtypes = [vp, C.POINTER(kc_batch_csr), pp]
+        L.kc_batch_free.argtypes = [vp]
+        L.kc_batch_free.restype = None
+        L.kc_batch_peel.argtypes = [vp, vp, C.POINTER(kc_params), u32p, u32p, C.POINTER(kc_stats)]
+        L.kc_batch_download_order.argtypes = [vp, vp, u32p]
         L.kc_graph_download_debug.argtypes = [vp, vp, u32p, C.c_uint32, u32p, u32p, C.c_uint32]
         _hip = L
     return _hip

@@ -1,0 +1,221 @@
+// batch_kernel.h — batched peeling: many independent graphs packed into one CSR, peeled by one
+// persistent kernel in which every workgroup owns one graph at a time.
+//
+// Same method as the single-graph kernel (reference src/ours.cc:2-160, per graph), but graphs are
+// independent, so a workgroup runs a graph's whole level loop with workgroup barriers only: no
+// grid barrier, no cross-workgroup queue. Workgroups claim graphs from a device counter in the
+// order the host gives (largest first, so the long graphs start early), each with its own level
+// counter.
+//
+// Within a graph, each round's frontier (a range of the graph's order[] segment) is processed in
+// tiles of BLOCK vertices: a workgroup-wide prefix sum of their degrees spreads the tile's arcs
+// evenly over every thread of the workgroup (a hub's arcs over all waves: the reference's
+// block-per-vertex fallback, applied to whole tiles), with kU arcs per thread in flight.
+// Crossings are appended to the graph's order[] through an LDS counter and form the next round.
+// The next level is the smallest degree left above the level, so empty levels are not scanned.
+//
+// Packed layout (include/kcore.h kc_batch_csr): graph g owns vertices [vtx_off[g], vtx_off[g+1])
+// of the packed vertex arrays (local ids 0..V_g-1), arcs [arc_off[g], arc_off[g+1]) of the packed
+// neighbour array (local ids), and V_g+1 local offsets starting at off[vtx_off[g] + g].
+#pragma once
+#include "device_common.h"
+
+namespace kcdev {
+
+struct BatchArgs {
+    const uint64_t* vtx_off;  // G+1
+    const uint64_t* arc_off;  // G+1
+    const uint32_t* off;      // sum(V_g + 1): local offsets
+    const uint32_t* nbr;      // local neighbour ids
+    uint32_t* core;           // packed working degrees -> core numbers (copied from the input
+                              // degrees by a separate kernel before the launch)
+    uint32_t* order;          // packed per-graph peel order
+    uint32_t* kmax;           // G
+    const uint32_t* sched;    // G graph ids, in claim order
+    uint32_t* ctrl;           // [0] next claim, [64] levels, [128] rounds, [192] error
+    uint32_t G;
+    uint32_t flags;
+};
+
+constexpr uint32_t kBU = 4;  // arcs per thread in flight
+
+template <int BLOCK>
+struct BatchState {
+    static constexpr int WPB = BLOCK / 64;
+    uint32_t tincl[BLOCK];  // tile: inclusive degree prefix
+    uint32_t tstart[BLOCK]; // tile: adjacency start
+    uint32_t wtot[WPB + 1];
+    uint32_t tail;          // order[] entries of this graph so far
+    uint32_t mn;            // smallest degree above the level
+    uint32_t g;             // current graph
+};
+
+// Workgroup-wide inclusive prefix sum (all threads call); *total receives the sum.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_incl_scan(BatchState<BLOCK>& S, uint32_t x, uint32_t* total) {
+    constexpr uint32_t WPB = BLOCK / 64;
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    const uint32_t inc = wave_incl_scan(x);
+    if (lane == 63) S.wtot[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t t = lane < WPB ? S.wtot[lane] : 0;
+        const uint32_t ti = wave_incl_scan(t);
+        if (lane < WPB) S.wtot[lane] = ti - t;
+        if (lane == WPB - 1) S.wtot[WPB] = ti;
+    }
+    __syncthreads();
+    *total = S.wtot[WPB];
+    return inc + S.wtot[w];
+}
+
+// Append the lanes' u (where pred) to the graph's order[] (LDS tail, one atomic per wave).
+template <int BLOCK>
+__device__ __forceinline__ void batch_append(BatchState<BLOCK>& S, uint32_t* order_g, bool pred, uint32_t u) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = __hip_atomic_fetch_add(&S.tail, (uint32_t)__popcll(m), __ATOMIC_RELAXED, KC_WG);
+    base = bcast_first(base);
+    if (pred) st_rlx(order_g + base + rank_below(m), u);
+}
+
+template <int BLOCK>
+__device__ void batch_peel_graph(const BatchArgs& a, BatchState<BLOCK>& S, uint32_t g) {
+    const uint32_t tid = threadIdx.x;
+    const uint64_t vb = a.vtx_off[g];
+    const uint32_t Vg = (uint32_t)(a.vtx_off[g + 1] - vb);
+    const uint32_t* off = a.off + vb + g;
+    const uint32_t* nbr = a.nbr + a.arc_off[g];
+    // the working degrees were copied from the input degrees by a kernel launched before this
+    // one, so within this launch a graph's degrees only ever decrease (plain prechecks may read
+    // stale values, but never too-low ones)
+    uint32_t* core = a.core + vb;
+    uint32_t* order = a.order + vb;
+    if (tid == 0) {
+        S.tail = 0;
+        S.mn = kSent;
+    }
+    __syncthreads();
+    uint32_t L = 0, levels = 0, rounds = 0;
+    while (true) {
+        // select: every v with core[v] == L (SC1 loads see other waves' atomics)
+        uint32_t mn = kSent;
+        const uint32_t lstart = S.tail;
+        for (uint32_t i0 = 0; i0 < Vg; i0 += BLOCK * kBU) {
+            uint32_t d[kBU];
+#pragma unroll
+            for (uint32_t k = 0; k < kBU; ++k) {
+                const uint32_t i = i0 + k * BLOCK + tid;
+                d[k] = i < Vg ? ld_rlx(core + i) : kSent;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kBU; ++k) {
+                if (d[k] > L && d[k] < mn) mn = d[k];
+                batch_append(S, order, d[k] == L, i0 + k * BLOCK + tid);
+            }
+        }
+        levels++;
+        drain_memory();
+        __syncthreads();
+        // rounds: process [h, t), crossings append at t and form the next round
+        uint32_t h = lstart;
+        uint32_t t = S.tail;
+        while (h < t) {
+            rounds++;
+            for (uint32_t tb = h; tb < t; tb += BLOCK) {
+                const uint32_t i = tb + tid;
+                uint32_t s = 0, dv = 0;
+                if (i < t) {
+                    const uint32_t v = ld_rlx(order + i);
+                    s = off[v];
+                    dv = off[v + 1] - s;
+                }
+                uint32_t T;
+                const uint32_t inc = block_incl_scan(S, dv, &T);
+                S.tincl[tid] = inc;
+                S.tstart[tid] = s;
+                __syncthreads();
+                for (uint32_t jb = 0; jb < T; jb += BLOCK * kBU) {
+                    bool act[kBU];
+                    uint32_t u[kBU];
+#pragma unroll
+                    for (uint32_t k = 0; k < kBU; ++k) {
+                        const uint32_t j = jb + k * BLOCK + tid;
+                        act[k] = j < T;
+                        u[k] = 0;
+                        if (act[k]) {
+                            uint32_t o = 0;
+#pragma unroll
+                            for (uint32_t step = BLOCK / 2; step; step >>= 1)
+                                if (S.tincl[o + step - 1] <= j) o += step;
+                            const uint32_t ex = o ? S.tincl[o - 1] : 0;
+                            u[k] = nbr[S.tstart[o] + (j - ex)];
+                        }
+                    }
+                    uint32_t du[kBU], old[kBU];
+#pragma unroll
+                    for (uint32_t k = 0; k < kBU; ++k) du[k] = act[k] ? core[u[k]] : 0;
+#pragma unroll
+                    for (uint32_t k = 0; k < kBU; ++k)
+                        old[k] = du[k] > L ? __hip_atomic_fetch_sub(core + u[k], 1u, __ATOMIC_RELAXED, KC_AGENT) : 0;
+#pragma unroll
+                    for (uint32_t k = 0; k < kBU; ++k) {
+                        const bool cross = du[k] > L && old[k] == L + 1;
+                        if (du[k] > L && old[k] <= L)
+                            __hip_atomic_fetch_add(core + u[k], 1u, __ATOMIC_RELAXED, KC_AGENT);
+                        else if (du[k] > L && old[k] > L + 1 && old[k] - 1 < mn)
+                            mn = old[k] - 1;
+                        batch_append(S, order, cross, u[k]);
+                    }
+                }
+                __syncthreads();  // the tile arrays are rewritten by the next tile
+            }
+            drain_memory();
+            __syncthreads();
+            h = t;
+            t = S.tail;
+        }
+        // level end
+        mn = wave_min(mn);
+        if (lane_id() == 0) __hip_atomic_fetch_min(&S.mn, mn, __ATOMIC_RELAXED, KC_WG);
+        __syncthreads();
+        const uint32_t tl = S.tail;
+        const uint32_t m = S.mn;
+        __syncthreads();
+        if (tl >= Vg) break;
+        uint32_t next = L + 1;
+        if (!(a.flags & 1u) && m != kSent && m > next) next = m;
+        if (next >= Vg) {  // a core number is below V_g on a simple graph: no progress
+            if (tid == 0) __hip_atomic_fetch_max(a.ctrl + 192, 2u, __ATOMIC_RELAXED, KC_AGENT);
+            break;
+        }
+        L = next;
+        if (tid == 0) S.mn = kSent;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        st_rlx(a.kmax + g, Vg ? L : 0);
+        add_rlx(a.ctrl + 64, levels);
+        add_rlx(a.ctrl + 128, rounds);
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void peel_batch(BatchArgs a) {
+    __shared__ BatchState<BLOCK> S;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t c = add_rlx(a.ctrl, 1u);
+            S.g = c < a.G ? a.sched[c] : kSent;
+        }
+        __syncthreads();
+        const uint32_t g = S.g;
+        __syncthreads();
+        if (g == kSent) break;
+        batch_peel_graph<BLOCK>(a, S, g);
+        __syncthreads();
+    }
+}
+
+}  // namespace kcdev

@@ -142,3 +142,123 @@ int kcg_write_edgelist(const kcg_graph* g, const char* path) {
 uint64_t kcg_splitmix64(uint64_t x) { return kcg::splitmix64(x); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// batches
+
+#include <omp.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+struct kcg_batch {
+    uint32_t G = 0;
+    std::vector<uint64_t> vtx_off, arc_off;
+    std::vector<uint32_t> offsets, neighbors, degrees;
+};
+
+static int pack_batch(const std::vector<const kcg::Csr*>& gs, kcg_batch** out) {
+    kcg_batch* b = new (std::nothrow) kcg_batch;
+    if (!b) return KCG_ERR_ARG;
+    const uint32_t G = (uint32_t)gs.size();
+    b->G = G;
+    b->vtx_off.assign((size_t)G + 1, 0);
+    b->arc_off.assign((size_t)G + 1, 0);
+    for (uint32_t g = 0; g < G; ++g) {
+        b->vtx_off[g + 1] = b->vtx_off[g] + gs[g]->V;
+        b->arc_off[g + 1] = b->arc_off[g] + gs[g]->arcs;
+    }
+    b->offsets.resize(b->vtx_off[G] + G);
+    b->neighbors.resize(b->arc_off[G]);
+    b->degrees.resize(b->vtx_off[G]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t g = 0; g < (int64_t)G; ++g) {
+        const kcg::Csr& c = *gs[g];
+        std::memcpy(b->offsets.data() + b->vtx_off[g] + g, c.offsets.data(), ((size_t)c.V + 1) * 4);
+        if (c.arcs) std::memcpy(b->neighbors.data() + b->arc_off[g], c.neighbors.data(), c.arcs * 4);
+        if (c.V) std::memcpy(b->degrees.data() + b->vtx_off[g], c.degrees.data(), (size_t)c.V * 4);
+    }
+    *out = b;
+    return KCG_OK;
+}
+
+extern "C" {
+
+int kcg_gen_batch(uint32_t G, uint32_t nmin, uint32_t nmax, double dmin, double dmax, double gamma,
+                  uint64_t seed, int nthreads, kcg_batch** out) {
+    if (!out || G == 0 || nmin < 4 || nmax < nmin || dmin <= 0 || dmax < dmin) return KCG_ERR_ARG;
+    std::vector<kcg::Csr> gs(G);
+    std::vector<int> rc(G, 0);
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    // graphs are generated in parallel, one thread each; each graph's draws depend only on (seed, i)
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+    for (int64_t i = 0; i < (int64_t)G; ++i) {
+        const uint64_t gs_seed = kcg::splitmix64(seed * 0x9E3779B97F4A7C15ull + (uint64_t)i);
+        const uint64_t r1 = kcg::splitmix64(gs_seed ^ 0x51ED27A5u), r2 = kcg::splitmix64(r1);
+        const double u1 = (double)(r1 >> 11) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(r2 >> 11) * (1.0 / 9007199254740992.0);
+        const uint32_t n = (uint32_t)std::llround(std::exp(std::log((double)nmin) +
+                                                          u1 * (std::log((double)nmax) - std::log((double)nmin))));
+        const double avg = dmin + u2 * (dmax - dmin);
+        if (i % 2 == 0) {
+            uint64_t m = (uint64_t)std::llround(n * avg / 2.0);
+            const uint64_t cap = (uint64_t)n * (n - 1) / 4;
+            if (m > cap) m = cap;
+            rc[i] = kcg::gen_er(n, m, gs_seed, 1, gs[i]);
+        } else {
+            const double wmax = std::min(2800.0, n / 2.0);
+            rc[i] = kcg::gen_chung_lu(n, gamma, 1.0, wmax, avg, gs_seed, 1, gs[i]);
+        }
+    }
+    for (uint32_t i = 0; i < G; ++i)
+        if (rc[i]) return rc[i];
+    std::vector<const kcg::Csr*> ptrs(G);
+    for (uint32_t i = 0; i < G; ++i) ptrs[i] = &gs[i];
+    return pack_batch(ptrs, out);
+}
+
+int kcg_batch_from_graphs(const kcg_graph* const* graphs, uint32_t G, kcg_batch** out) {
+    if (!graphs || !out || G == 0) return KCG_ERR_ARG;
+    std::vector<const kcg::Csr*> ptrs(G);
+    for (uint32_t i = 0; i < G; ++i) {
+        if (!graphs[i]) return KCG_ERR_ARG;
+        ptrs[i] = &graphs[i]->csr;
+    }
+    return pack_batch(ptrs, out);
+}
+
+void kcg_batch_free(kcg_batch* b) { delete b; }
+
+void kcg_batch_info(const kcg_batch* b, uint32_t* G, uint64_t* total_vertices, uint64_t* total_arcs) {
+    if (G) *G = b->G;
+    if (total_vertices) *total_vertices = b->vtx_off[b->G];
+    if (total_arcs) *total_arcs = b->arc_off[b->G];
+}
+
+void kcg_batch_arrays(const kcg_batch* b, const uint64_t** vtx_off, const uint64_t** arc_off,
+                      const uint32_t** offsets, const uint32_t** neighbors, const uint32_t** degrees) {
+    if (vtx_off) *vtx_off = b->vtx_off.data();
+    if (arc_off) *arc_off = b->arc_off.data();
+    if (offsets) *offsets = b->offsets.data();
+    if (neighbors) *neighbors = b->neighbors.data();
+    if (degrees) *degrees = b->degrees.data();
+}
+
+int kcg_batch_graph(const kcg_batch* b, uint32_t g, kcg_graph** out) {
+    if (!b || !out || g >= b->G) return KCG_ERR_ARG;
+    const uint32_t V = (uint32_t)(b->vtx_off[g + 1] - b->vtx_off[g]);
+    const uint64_t arcs = b->arc_off[g + 1] - b->arc_off[g];
+    kcg::Csr c;
+    c.V = V;
+    c.arcs = arcs;
+    const uint32_t* off = b->offsets.data() + b->vtx_off[g] + g;
+    c.offsets.assign(off, off + (size_t)V + 1);
+    c.neighbors.assign(b->neighbors.data() + b->arc_off[g], b->neighbors.data() + b->arc_off[g] + arcs);
+    c.degrees.assign(b->degrees.data() + b->vtx_off[g], b->degrees.data() + b->vtx_off[g] + V);
+    c.dmax = 0;
+    for (uint32_t v = 0; v < V; ++v) c.dmax = std::max(c.dmax, c.degrees[v]);
+    return wrap(std::move(c), out);
+}
+
+}  // extern "C"

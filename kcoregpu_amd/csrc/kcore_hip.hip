@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/kcore.h"
+#include "batch_kernel.h"
 #include "peel_kernel.h"
 
 using namespace kcdev;
@@ -35,6 +36,45 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t s = gridDim.x * blockDim.x;
     for (; i < nvec; i += s) dst[i] = src[i];
+}
+
+// CSR checks on the device (one pass over the arrays): offsets non-decreasing and every
+// neighbour id inside its graph. Sets *bad to 1 on any violation.
+__global__ void validate_csr(const uint32_t* off, const uint32_t* nbr, uint32_t V, uint64_t arcs, uint32_t* bad) {
+    const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < arcs; i += s)
+        if (nbr[i] >= V) *bad = 1;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+        if (off[v + 1] < off[v]) *bad = 1;
+}
+
+// one workgroup per graph (grid-strided): offsets start at 0, end at the graph's arc count,
+// never decrease, and every local id is below V_g
+__global__ void validate_batch(const uint64_t* vtx_off, const uint64_t* arc_off, const uint32_t* off,
+                               const uint32_t* nbr, uint32_t G, uint32_t* bad) {
+    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+        const uint64_t vb = vtx_off[g], ab = arc_off[g];
+        const uint64_t V = vtx_off[g + 1] - vb, A = arc_off[g + 1] - ab;
+        const uint32_t* o = off + vb + g;
+        if (V >= (1ull << 32) || A >= (1ull << 32) || vtx_off[g + 1] < vb || arc_off[g + 1] < ab) {
+            *bad = 1;
+            continue;
+        }
+        if (threadIdx.x == 0 && (o[0] != 0 || o[V] != A)) *bad = 1;
+        for (uint64_t v = threadIdx.x; v < V; v += blockDim.x)
+            if (o[v + 1] < o[v]) *bad = 1;
+        for (uint64_t j = threadIdx.x; j < A; j += blockDim.x)
+            if (nbr[ab + j] >= V) *bad = 1;
+    }
+}
+
+__global__ void batch_degrees(const uint64_t* vtx_off, const uint32_t* off, uint32_t G, uint32_t* deg) {
+    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+        const uint64_t vb = vtx_off[g];
+        const uint64_t V = vtx_off[g + 1] - vb;
+        const uint32_t* o = off + vb + g;
+        for (uint64_t v = threadIdx.x; v < V; v += blockDim.x) deg[vb + v] = o[v + 1] - o[v];
+    }
 }
 
 __global__ void selftest_scan_kernel(const uint32_t* in, uint32_t* out) {
@@ -104,6 +144,18 @@ static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out
     // A plain launch: the kernel's elastic start (register_workgroup) only lets workgroups
     // that are resident together take part, so it needs no co-residency guarantee.
     peel_persistent<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    KC_TRY(hipGetLastError());
+    return KC_OK;
+}
+
+template <int BLOCK>
+static kc_status launch_batch(kc_ctx* c, const BatchArgs& a) {
+    int per_cu = 0;
+    KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_batch<BLOCK>, BLOCK, 0));
+    if (per_cu < 1) per_cu = 1;
+    // graphs are independent: any grid size is correct; more workgroups than graphs is waste
+    uint32_t grid = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)c->cus, a.G);
+    peel_batch<BLOCK><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
     KC_TRY(hipGetLastError());
     return KC_OK;
 }
@@ -225,8 +277,17 @@ kc_status kc_graph_upload(kc_ctx* c, const kc_csr* h, kc_graph** out) {
     } else {
         degrees_from_offsets<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_deg0, g->V, g->padV);
     }
+    // reject a malformed CSR before any kernel indexes with it
+    fill_kernel<uint32_t><<<1, 64, 0, c->stream>>>(g->d_ctrl, 0u, 1);
+    validate_csr<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_nbr, g->V, g->arcs, g->d_ctrl);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    uint32_t bad = 0;
+    if ((e = hipMemcpyAsync(&bad, g->d_ctrl, 4, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e);
+    if (bad) {
+        kc_graph_free(g);
+        return KC_ERR_ARG;
+    }
     *out = g;
     return KC_OK;
 }
@@ -434,6 +495,175 @@ kc_status kc_selftest(kc_ctx* c) {
         acc += h_in[i];
         if (h_out[i] != acc) return KC_ERR_DEVICE;
     }
+    return KC_OK;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// batches
+
+struct kc_batch {
+    kc_ctx* ctx = nullptr;
+    uint32_t G = 0;
+    uint64_t nv = 0, na = 0, padv = 0;
+    uint64_t *d_vtx = nullptr, *d_arc = nullptr;
+    uint32_t *d_off = nullptr, *d_nbr = nullptr, *d_deg0 = nullptr, *d_core = nullptr, *d_order = nullptr;
+    uint32_t *d_kmax = nullptr, *d_sched = nullptr, *d_ctrl = nullptr;
+    std::vector<uint64_t> vtx_off, arc_off;
+    bool peeled = false;
+};
+
+void kc_batch_free(kc_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (void* p : {(void*)b->d_vtx, (void*)b->d_arc, (void*)b->d_off, (void*)b->d_nbr, (void*)b->d_deg0,
+                    (void*)b->d_core, (void*)b->d_order, (void*)b->d_kmax, (void*)b->d_sched, (void*)b->d_ctrl})
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+kc_status kc_batch_upload(kc_ctx* c, const kc_batch_csr* h, kc_batch** out) {
+    if (!c || !h || !out || h->G == 0 || !h->vtx_off || !h->arc_off || !h->offsets ||
+        (h->total_arcs && !h->neighbors))
+        return KC_ERR_ARG;
+    if (h->vtx_off[0] != 0 || h->arc_off[0] != 0 || h->vtx_off[h->G] != h->total_vertices ||
+        h->arc_off[h->G] != h->total_arcs)
+        return KC_ERR_ARG;
+    // graph ranges must be non-decreasing before any device code indexes with them; graph
+    // sizes must fit the kernel's 32-bit per-graph indices
+    for (uint32_t g = 0; g < h->G; ++g) {
+        if (h->vtx_off[g + 1] < h->vtx_off[g] || h->arc_off[g + 1] < h->arc_off[g]) return KC_ERR_ARG;
+        if (h->vtx_off[g + 1] - h->vtx_off[g] >= (1ull << 30) || h->arc_off[g + 1] - h->arc_off[g] >= (1ull << 32))
+            return KC_ERR_CAPACITY;
+    }
+    KC_TRY(hipSetDevice(c->device));
+    kc_batch* b = new (std::nothrow) kc_batch;
+    if (!b) return KC_ERR_OOM;
+    b->ctx = c;
+    b->G = h->G;
+    b->nv = h->total_vertices;
+    b->na = h->total_arcs;
+    b->padv = (b->nv + 255) & ~255ull;
+    b->vtx_off.assign(h->vtx_off, h->vtx_off + h->G + 1);
+    b->arc_off.assign(h->arc_off, h->arc_off + h->G + 1);
+    // claim order: most arcs first, so the longest graphs start first
+    std::vector<uint32_t> sched(h->G);
+    for (uint32_t g = 0; g < h->G; ++g) sched[g] = g;
+    std::stable_sort(sched.begin(), sched.end(), [&](uint32_t x, uint32_t y) {
+        return h->arc_off[x + 1] - h->arc_off[x] > h->arc_off[y + 1] - h->arc_off[y];
+    });
+    auto fail = [&](hipError_t e) {
+        kc_batch_free(b);
+        return hip_status(e);
+    };
+    hipError_t e;
+#define KC_ALLOC(ptr, bytes) \
+    if ((e = hipMalloc(&(ptr), (bytes))) != hipSuccess) return fail(e)
+    KC_ALLOC(b->d_vtx, ((size_t)h->G + 1) * 8);
+    KC_ALLOC(b->d_arc, ((size_t)h->G + 1) * 8);
+    KC_ALLOC(b->d_off, (b->nv + h->G) * 4);
+    KC_ALLOC(b->d_nbr, std::max<uint64_t>(b->na, 1) * 4);
+    KC_ALLOC(b->d_deg0, b->padv * 4);
+    KC_ALLOC(b->d_core, b->padv * 4);
+    KC_ALLOC(b->d_order, b->padv * 4);
+    KC_ALLOC(b->d_kmax, (size_t)h->G * 4);
+    KC_ALLOC(b->d_sched, (size_t)h->G * 4);
+    KC_ALLOC(b->d_ctrl, 4096);
+#undef KC_ALLOC
+    hipStream_t st = c->stream;
+    if ((e = hipMemcpyAsync(b->d_vtx, h->vtx_off, ((size_t)h->G + 1) * 8, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(b->d_arc, h->arc_off, ((size_t)h->G + 1) * 8, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(b->d_off, h->offsets, (b->nv + h->G) * 4, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(b->d_sched, sched.data(), (size_t)h->G * 4, hipMemcpyHostToDevice, st)))
+        return fail(e);
+    if (b->na && (e = hipMemcpyAsync(b->d_nbr, h->neighbors, b->na * 4, hipMemcpyHostToDevice, st))) return fail(e);
+    fill_kernel<uint32_t><<<1, 64, 0, st>>>(b->d_ctrl, 0u, 1024);
+    validate_batch<<<1024, 256, 0, st>>>(b->d_vtx, b->d_arc, b->d_off, b->d_nbr, h->G, b->d_ctrl);
+    fill_kernel<uint32_t><<<64, 256, 0, st>>>(b->d_deg0, kSent, b->padv);
+    if (h->degrees) {
+        if ((e = hipMemcpyAsync(b->d_deg0, h->degrees, b->nv * 4, hipMemcpyHostToDevice, st))) return fail(e);
+    } else {
+        batch_degrees<<<1024, 256, 0, st>>>(b->d_vtx, b->d_off, h->G, b->d_deg0);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    uint32_t bad = 0;
+    if ((e = hipMemcpyAsync(&bad, b->d_ctrl, 4, hipMemcpyDeviceToHost, st)) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
+    if (bad) {
+        kc_batch_free(b);
+        return KC_ERR_ARG;
+    }
+    *out = b;
+    return KC_OK;
+}
+
+kc_status kc_batch_peel(kc_ctx* c, kc_batch* b, const kc_params* p, uint32_t* core_out, uint32_t* kmax_out,
+                        kc_stats* stt) {
+    if (!c || !b || b->ctx != c) return KC_ERR_ARG;
+    KC_TRY(hipSetDevice(c->device));
+    kc_params prm{};
+    if (p) prm = *p;
+    const uint32_t block = prm.block ? prm.block : 512;
+    if (block != 256 && block != 512 && block != 1024) return KC_ERR_ARG;
+    BatchArgs a{};
+    a.vtx_off = b->d_vtx;
+    a.arc_off = b->d_arc;
+    a.off = b->d_off;
+    a.nbr = b->d_nbr;
+    a.core = b->d_core;
+    a.order = b->d_order;
+    a.kmax = b->d_kmax;
+    a.sched = b->d_sched;
+    a.ctrl = b->d_ctrl;
+    a.G = b->G;
+    a.flags = prm.flags;
+    KC_TRY(hipEventRecord(c->ev0, c->stream));
+    fill_kernel<uint32_t><<<1, 256, 0, c->stream>>>(b->d_ctrl, 0u, 1024);
+    copy_kernel<<<2048, 256, 0, c->stream>>>(reinterpret_cast<const uint4*>(b->d_deg0),
+                                             reinterpret_cast<uint4*>(b->d_core), (uint32_t)(b->padv / 4));
+    KC_TRY(hipGetLastError());
+    KC_TRY(hipEventRecord(c->evk0, c->stream));
+    kc_status s;
+    switch (block) {
+        case 256: s = launch_batch<256>(c, a); break;
+        case 512: s = launch_batch<512>(c, a); break;
+        default: s = launch_batch<1024>(c, a); break;
+    }
+    if (s != KC_OK) return s;
+    KC_TRY(hipEventRecord(c->evk1, c->stream));
+    KC_TRY(hipEventRecord(c->ev1, c->stream));
+    uint32_t ctl[256];
+    KC_TRY(hipMemcpyAsync(ctl, b->d_ctrl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+    KC_TRY(hipStreamSynchronize(c->stream));
+    if (ctl[192]) return KC_ERR_NO_PROGRESS;
+    float ms = 0, msk = 0;
+    KC_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    KC_TRY(hipEventElapsedTime(&msk, c->evk0, c->evk1));
+    b->peeled = true;
+    if (core_out && b->nv)
+        KC_TRY(hipMemcpyAsync(core_out, b->d_core, b->nv * 4, hipMemcpyDeviceToHost, c->stream));
+    if (kmax_out) KC_TRY(hipMemcpyAsync(kmax_out, b->d_kmax, (size_t)b->G * 4, hipMemcpyDeviceToHost, c->stream));
+    KC_TRY(hipStreamSynchronize(c->stream));
+    if (stt) {
+        memset(stt, 0, sizeof(*stt));
+        stt->ms_peel = ms;
+        stt->ms_kernel = msk;
+        stt->levels = ctl[64];
+        stt->subrounds = ctl[128];
+        stt->block = block;
+        // scan term: each graph scans its V_g entries once per level it runs
+        stt->bytes_edge = 16ull * b->nv + 20ull * (b->na / 2);
+        stt->bytes_alg = stt->bytes_edge;  // the per-graph scan term is added by the caller
+    }
+    return KC_OK;
+}
+
+kc_status kc_batch_download_order(kc_ctx* c, const kc_batch* b, uint32_t* order_out) {
+    if (!c || !b || !order_out || !b->peeled) return KC_ERR_ARG;
+    KC_TRY(hipSetDevice(c->device));
+    if (b->nv) KC_TRY(hipMemcpyAsync(order_out, b->d_order, b->nv * 4, hipMemcpyDeviceToHost, c->stream));
+    KC_TRY(hipStreamSynchronize(c->stream));
     return KC_OK;
 }
 
