@@ -155,6 +155,43 @@ kc_status kc_batch_peel(kc_ctx* ctx, kc_batch* b, const kc_params* p, uint32_t* 
 /* Per-graph peel order, packed like the vertex arrays (local ids). */
 kc_status kc_batch_download_order(kc_ctx* ctx, const kc_batch* b, uint32_t* order_out);
 
+/* ---- served peeling (no reference counterpart) ----
+ * A server keeps one persistent kernel resident on the device. Each request is a graph as a host
+ * CSR (degrees required); it is admitted, in arrival order, when a ring slot is free and the
+ * HBM arena has room for its footprint (offsets + neighbours + three vertex arrays, each rounded
+ * to 256 bytes: 8 m + 16 n bytes for n vertices and m undirected edges). Its CSR is copied into
+ * its region on a copy stream and published in the ring; an idle workgroup of the kernel takes it
+ * at once and peels it alone (requests join and leave the running kernel individually).
+ * kc_serve_poll reports finished requests after their core numbers have reached core_out and
+ * frees their regions. The CSR arrays and core_out must stay valid until the request's
+ * completion is returned. Not thread-safe: one host thread drives a server. */
+typedef struct kc_server kc_server;
+typedef struct {
+    uint64_t hbm_budget_bytes; /* arena size; 0 = 8 GiB */
+    uint32_t ring_slots;       /* most requests admitted at once; 0 = 1024 */
+    uint32_t block;            /* 256, 512 or 1024 threads per workgroup; 0 = 256 */
+    uint32_t workgroups;       /* resident serving workgroups; 0 = 2 per CU (256) or 1 per CU */
+    uint32_t flags;            /* KC_FLAG_NO_LEVEL_SKIP */
+} kc_serve_cfg;
+typedef struct {
+    uint64_t req_id;
+    kc_status status;          /* KC_OK, or KC_ERR_NO_PROGRESS when the level loop overran */
+    uint32_t kmax, levels, rounds;
+    double t_submit, t_admit, t_done; /* kc_serve_clock() seconds */
+} kc_completion;
+kc_status kc_serve_open(kc_ctx* ctx, const kc_serve_cfg* cfg, kc_server** out);
+/* KC_ERR_CAPACITY: V >= 2^30, arcs >= 2^32 or a footprint above the whole arena. */
+kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32_t* core_out);
+/* Writes up to cap completions; *n_out = how many. Also admits waiting requests. */
+kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t* n_out);
+kc_status kc_serve_pending(const kc_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* arena_used);
+/* Stops the kernel after every admitted request has run; KC_ERR_DEVICE if it does not exit. */
+kc_status kc_serve_close(kc_server* s);
+double kc_serve_clock(void);
+/* Pinned host memory (fast asynchronous copies) for request buffers. */
+kc_status kc_host_alloc(uint64_t bytes, void** out);
+kc_status kc_host_free(void* p);
+
 /* Diagnostics (no reference counterpart). kc_selftest checks the wave64 DPP prefix sum the
  * kernels use against a host prefix sum on one wavefront (KC_OK when they agree).
  * kc_graph_download_trace returns the barrier timestamps of the last peel run with

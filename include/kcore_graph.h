@@ -77,6 +77,17 @@ int kcg_gen_chung_lu(uint32_t n, double gamma, double wmin, double wmax, double 
 int kcg_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c,
                  uint64_t seed, int nthreads, kcg_graph** out);
 
+/* Served-workload requests: request i of stream `seed` asks for m edges, m lognormal with the
+ * given median and sigma (Box-Muller on counter-based draws), truncated to [mmin, mmax] by
+ * redrawing, and a seeded coin flip picks the family (kind 0: R-MAT (0.57, 0.19, 0.19) on the
+ * smallest 2^scale >= m/8 vertices with m draws; kind 1: Chung-Lu, exponent 2.1, n = m/6,
+ * average degree 2m/n, weights clipped to [1, min(2800, n/2)]). Deduplication and self-loop
+ * removal leave somewhat fewer than m distinct edges. */
+int kcg_request_shape(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
+                      uint64_t* m_out, int* kind_out);
+int kcg_gen_request(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
+                    int nthreads, kcg_graph** out, int* kind_out);
+
 /* Write the pair list a generator would feed the loader as a 'u v' text edge list
  * (one line per undirected edge u<v of the built CSR). */
 int kcg_write_edgelist(const kcg_graph* g, const char* path);

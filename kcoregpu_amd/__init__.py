@@ -15,6 +15,7 @@ Every compute call goes through the HIP library; nothing here computes core numb
 from __future__ import annotations
 
 import atexit
+import time
 import ctypes as C
 import os
 import weakref
@@ -81,9 +82,9 @@ class Graph:
             self._h = C.c_void_p(0)
 
     @staticmethod
-    def _make(fn, *args) -> "Graph":
+    def _make(fn, *args, kind_ptr=None) -> "Graph":
         out = C.c_void_p()
-        st = fn(*args, C.byref(out))
+        st = fn(*args, C.byref(out)) if kind_ptr is None else fn(*args, C.byref(out), kind_ptr)
         if st != 0:
             raise KCoreError(st, KCG_ERRORS.get(st, "graph error"))
         return Graph(out.value)
@@ -140,6 +141,26 @@ class Graph:
              c: float = 0.19, ndraws: int | None = None, nthreads: int = 0) -> "Graph":
         nd = ndraws if ndraws is not None else edge_factor << scale
         return cls._make(N.host().kcg_gen_rmat, scale, nd, a, b, c, seed, nthreads)
+
+    @classmethod
+    def request(cls, seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.5,
+                mmin: int = 1000, mmax: int = 50_000_000, nthreads: int = 0) -> "Graph":
+        """Request i of a served-workload stream (R-MAT or Chung-Lu by a seeded coin flip, edge
+        count lognormal around median_m); .kind is 0 for R-MAT, 1 for Chung-Lu."""
+        kind = C.c_int()
+        g = cls._make(N.host().kcg_gen_request, seed, i, median_m, sigma, mmin, mmax, nthreads,
+                      kind_ptr=C.byref(kind))
+        g.kind = kind.value
+        return g
+
+    @staticmethod
+    def request_shape(seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.5,
+                      mmin: int = 1000, mmax: int = 50_000_000):
+        m, kind = C.c_uint64(), C.c_int()
+        st = N.host().kcg_request_shape(seed, i, median_m, sigma, mmin, mmax, C.byref(m), C.byref(kind))
+        if st:
+            raise KCoreError(st, "kcg_request_shape")
+        return m.value, kind.value
 
     # ---- writers (reference src/graph.cpp:35-54, :126-140) ---------------------------
     def write_serialized(self, path: str) -> None:
@@ -292,6 +313,86 @@ class Engine:
         db = DeviceBatch(self, out, b)
         self._graphs.add(db)
         return db
+
+    def serve(self, **cfg) -> "Server":
+        srv = Server(self, **cfg)
+        self._graphs.add(srv)
+        return srv
+
+
+class Server:
+    """Served peeling (kc_serve_*): a resident persistent kernel that peels requests as they
+    arrive, each in its own region of an HBM arena. submit() never blocks on the GPU; poll()
+    returns finished requests with their core numbers."""
+
+    def __init__(self, eng: "Engine", hbm_budget: int = 0, ring_slots: int = 0, block: int = 0,
+                 workgroups: int = 0, flags: int = 0):
+        self.eng = eng
+        self._s = C.c_void_p()
+        cfg = N.kc_serve_cfg(hbm_budget, ring_slots, block, workgroups, flags)
+        _check(N.hip().kc_serve_open(eng._c, C.byref(cfg), C.byref(self._s)), "kc_serve_open")
+        self._live = {}  # req_id -> (graph, core buffer): kept alive until completion
+        self._next = 0
+
+    def submit(self, g: Graph, req_id: int | None = None) -> int:
+        if req_id is None:
+            req_id = self._next
+        self._next = max(self._next, req_id + 1)
+        if req_id in self._live:
+            raise ValueError(f"request id {req_id} is still in flight")
+        core = np.empty(g.V, np.uint32)
+        h = N.kc_csr(g.V, g.arcs, _u32(g.neighbors_offset),
+                     _u32(g.neighbors) if g.arcs else N.u32p(), _u32(g.degrees))
+        _check(N.hip().kc_serve_submit(self._s, req_id, C.byref(h), _u32(core)), "kc_serve_submit")
+        self._live[req_id] = (g, core)
+        return req_id
+
+    def poll(self, cap: int = 256) -> list:
+        buf = (N.kc_completion * cap)()
+        n = C.c_uint32()
+        _check(N.hip().kc_serve_poll(self._s, buf, cap, C.byref(n)), "kc_serve_poll")
+        out = []
+        for i in range(n.value):
+            c = buf[i]
+            _, core = self._live.pop(c.req_id)
+            out.append({"req_id": c.req_id, "status": c.status, "core": core, "kmax": c.kmax,
+                        "levels": c.levels, "rounds": c.rounds, "t_submit": c.t_submit,
+                        "t_admit": c.t_admit, "t_done": c.t_done})
+        return out
+
+    def pending(self):
+        q, f, used = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(N.hip().kc_serve_pending(self._s, C.byref(q), C.byref(f), C.byref(used)), "kc_serve_pending")
+        return q.value, f.value, used.value
+
+    def drain(self, timeout_s: float = 60.0) -> list:
+        """Polls until every submitted request has completed."""
+        out = []
+        t0 = time.monotonic()
+        while self._live:
+            out += self.poll()
+            if time.monotonic() - t0 > timeout_s:
+                raise TimeoutError(f"{len(self._live)} requests unfinished after {timeout_s} s")
+        return out
+
+    def close(self):
+        if self._s.value:
+            s, self._s = self._s, C.c_void_p(0)
+            _check(N.hip().kc_serve_close(s), "kc_serve_close")
+
+    free = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBatch:

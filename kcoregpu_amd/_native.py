@@ -62,6 +62,17 @@ class kc_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class kc_serve_cfg(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("ring_slots", C.c_uint32), ("block", C.c_uint32),
+                ("workgroups", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class kc_completion(C.Structure):
+    _fields_ = [("req_id", C.c_uint64), ("status", C.c_int), ("kmax", C.c_uint32),
+                ("levels", C.c_uint32), ("rounds", C.c_uint32), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
 def host() -> C.CDLL:
     global _host
     if _host is None:
@@ -101,6 +112,10 @@ def host() -> C.CDLL:
                                        C.POINTER(u32p), C.POINTER(u32p)]
         L.kcg_batch_arrays.restype = None
         L.kcg_batch_graph.argtypes = [vp, C.c_uint32, pp]
+        L.kcg_request_shape.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
+                                        C.c_uint64, u64p, C.POINTER(C.c_int)]
+        L.kcg_gen_request.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
+                                      C.c_uint64, C.c_int, pp, C.POINTER(C.c_int)]
         _host = L
     return _host
 
@@ -137,5 +152,14 @@ def hip() -> C.CDLL:
         L.kc_batch_peel.argtypes = [vp, vp, C.POINTER(kc_params), u32p, u32p, C.POINTER(kc_stats)]
         L.kc_batch_download_order.argtypes = [vp, vp, u32p]
         L.kc_graph_download_debug.argtypes = [vp, vp, u32p, C.c_uint32, u32p, u32p, C.c_uint32]
+        L.kc_serve_open.argtypes = [vp, C.POINTER(kc_serve_cfg), pp]
+        L.kc_serve_submit.argtypes = [vp, C.c_uint64, C.POINTER(kc_csr), u32p]
+        L.kc_serve_poll.argtypes = [vp, C.POINTER(kc_completion), C.c_uint32, u32p]
+        L.kc_serve_pending.argtypes = [vp, u32p, u32p, u64p]
+        L.kc_serve_close.argtypes = [vp]
+        L.kc_serve_clock.argtypes = []
+        L.kc_serve_clock.restype = C.c_double
+        L.kc_host_alloc.argtypes = [C.c_uint64, pp]
+        L.kc_host_free.argtypes = [vp]
         _hip = L
     return _hip

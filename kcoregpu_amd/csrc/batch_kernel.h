@@ -80,18 +80,14 @@ __device__ __forceinline__ void batch_append(BatchState<BLOCK>& S, uint32_t* ord
     if (pred) st_rlx(order_g + base + rank_below(m), u);
 }
 
+// Peel one graph with this workgroup: off[V+1], nbr (local ids), core[V] (already holding the
+// input degrees, with no stale copy in any cache: see the callers), order[V] (peel order).
+// Returns kmax; *levels / *rounds count scanned levels and processing rounds.
 template <int BLOCK>
-__device__ void batch_peel_graph(const BatchArgs& a, BatchState<BLOCK>& S, uint32_t g) {
+__device__ uint32_t wg_peel_graph(BatchState<BLOCK>& S, const uint32_t* off, const uint32_t* nbr,
+                                  uint32_t* core, uint32_t* order, uint32_t Vg, uint32_t flags,
+                                  uint32_t* levels_out, uint32_t* rounds_out, uint32_t* err) {
     const uint32_t tid = threadIdx.x;
-    const uint64_t vb = a.vtx_off[g];
-    const uint32_t Vg = (uint32_t)(a.vtx_off[g + 1] - vb);
-    const uint32_t* off = a.off + vb + g;
-    const uint32_t* nbr = a.nbr + a.arc_off[g];
-    // the working degrees were copied from the input degrees by a kernel launched before this
-    // one, so within this launch a graph's degrees only ever decrease (plain prechecks may read
-    // stale values, but never too-low ones)
-    uint32_t* core = a.core + vb;
-    uint32_t* order = a.order + vb;
     if (tid == 0) {
         S.tail = 0;
         S.mn = kSent;
@@ -185,19 +181,35 @@ __device__ void batch_peel_graph(const BatchArgs& a, BatchState<BLOCK>& S, uint3
         __syncthreads();
         if (tl >= Vg) break;
         uint32_t next = L + 1;
-        if (!(a.flags & 1u) && m != kSent && m > next) next = m;
+        if (!(flags & 1u) && m != kSent && m > next) next = m;
         if (next >= Vg) {  // a core number is below V_g on a simple graph: no progress
-            if (tid == 0) __hip_atomic_fetch_max(a.ctrl + 192, 2u, __ATOMIC_RELAXED, KC_AGENT);
+            if (tid == 0) *err = 2;
             break;
         }
         L = next;
         if (tid == 0) S.mn = kSent;
         __syncthreads();
     }
-    if (tid == 0) {
-        st_rlx(a.kmax + g, Vg ? L : 0);
+    *levels_out = levels;
+    *rounds_out = rounds;
+    return Vg ? L : 0;
+}
+
+template <int BLOCK>
+__device__ void batch_peel_graph(const BatchArgs& a, BatchState<BLOCK>& S, uint32_t g) {
+    const uint64_t vb = a.vtx_off[g];
+    const uint32_t Vg = (uint32_t)(a.vtx_off[g + 1] - vb);
+    // the working degrees were copied from the input degrees by a kernel launched before this
+    // one, so within this launch a graph's degrees only ever decrease (plain prechecks may read
+    // stale values, but never too-low ones)
+    uint32_t levels = 0, rounds = 0, err = 0;
+    const uint32_t km = wg_peel_graph<BLOCK>(S, a.off + vb + g, a.nbr + a.arc_off[g], a.core + vb, a.order + vb,
+                                             Vg, a.flags, &levels, &rounds, &err);
+    if (threadIdx.x == 0) {
+        st_rlx(a.kmax + g, km);
         add_rlx(a.ctrl + 64, levels);
         add_rlx(a.ctrl + 128, rounds);
+        if (err) __hip_atomic_fetch_max(a.ctrl + 192, err, __ATOMIC_RELAXED, KC_AGENT);
     }
 }
 

@@ -218,6 +218,56 @@ int kcg_gen_batch(uint32_t G, uint32_t nmin, uint32_t nmax, double dmin, double 
     return pack_batch(ptrs, out);
 }
 
+int kcg_request_shape(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
+                      uint64_t* m_out, int* kind_out) {
+    if (!m_out || !kind_out || median_m <= 0 || sigma < 0 || mmin == 0 || mmax < mmin) return KCG_ERR_ARG;
+    const uint64_t key = kcg::splitmix64(seed * 0x9E3779B97F4A7C15ull + i);
+    auto unif = [&](uint64_t j) {  // (0, 1]
+        return ((double)(kcg::splitmix64(key + (j + 1) * 0x9E3779B97F4A7C15ull) >> 11) + 1.0) *
+               (1.0 / 9007199254740992.0);
+    };
+    // lognormal by Box-Muller, truncated to [mmin, mmax] by redrawing (at most 64 tries, then clamped)
+    double m = median_m;
+    for (uint64_t t = 0; t < 64; ++t) {
+        const double z = std::sqrt(-2.0 * std::log(unif(2 * t))) * std::cos(6.283185307179586 * unif(2 * t + 1));
+        m = median_m * std::exp(sigma * z);
+        if (m >= (double)mmin && m <= (double)mmax) break;
+        m = std::min(std::max(m, (double)mmin), (double)mmax);
+    }
+    *m_out = (uint64_t)std::llround(m);
+    *kind_out = (int)(kcg::splitmix64(key ^ 0xC0FFEEull) & 1);
+    return KCG_OK;
+}
+
+int kcg_gen_request(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
+                    int nthreads, kcg_graph** out, int* kind_out) {
+    uint64_t m;
+    int kind;
+    int rc = kcg_request_shape(seed, i, median_m, sigma, mmin, mmax, &m, &kind);
+    if (rc) return rc;
+    if (!out) return KCG_ERR_ARG;
+    const uint64_t gseed = kcg::splitmix64(seed * 0x9E3779B97F4A7C15ull + i) ^ 0x5EEDull;
+    auto* g = new kcg_graph;
+    if (kind == 0) {
+        // R-MAT, edge factor 8: 2^scale >= m/8 vertices, m draws (a, b, c) = (0.57, 0.19, 0.19)
+        uint32_t scale = 6;
+        while (scale < 30 && ((uint64_t)8 << scale) < m) ++scale;
+        rc = kcg::gen_rmat(scale, m, 0.57, 0.19, 0.19, gseed, nthreads, g->csr);
+    } else {
+        // Chung-Lu, exponent 2.1, average degree 12 -> n = m/6, weights clipped to [1, min(2800, n/2)]
+        const uint32_t n = (uint32_t)std::max<uint64_t>(64, (m + 3) / 6);
+        const double avg = 2.0 * (double)m / n;
+        rc = kcg::gen_chung_lu(n, 2.1, 1.0, std::min(2800.0, n / 2.0), avg, gseed, nthreads, g->csr);
+    }
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    if (kind_out) *kind_out = kind;
+    return KCG_OK;
+}
+
 int kcg_batch_from_graphs(const kcg_graph* const* graphs, uint32_t G, kcg_batch** out) {
     if (!graphs || !out || G == 0) return KCG_ERR_ARG;
     std::vector<const kcg::Csr*> ptrs(G);

@@ -11,6 +11,7 @@
 
 #include "../../include/kcore.h"
 #include "batch_kernel.h"
+#include "host_internal.h"
 #include "peel_kernel.h"
 
 using namespace kcdev;
@@ -88,15 +89,6 @@ constexpr uint32_t kMaxGrid = 4096;
 
 }  // namespace
 
-struct kc_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
-    int cus = 0;
-    uint64_t wall_rate_khz = 100000;
-    uint64_t wait_limit_ms = 20000;  // one device-side wait; KCORE_WAIT_LIMIT_MS overrides
-};
-
 struct kc_graph {
     kc_ctx* ctx = nullptr;
     uint32_t V = 0, padV = 0, dmax = 0;
@@ -119,18 +111,6 @@ struct kc_graph {
     uint32_t last_ntrace = 0;
     bool peeled = false;
 };
-
-static kc_status hip_status(hipError_t e) {
-    if (e == hipSuccess) return KC_OK;
-    if (e == hipErrorOutOfMemory) return KC_ERR_OOM;
-    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return KC_ERR_NO_DEVICE;
-    return KC_ERR_HIP;
-}
-#define KC_TRY(x)                                    \
-    do {                                             \
-        hipError_t e_ = (x);                         \
-        if (e_ != hipSuccess) return hip_status(e_); \
-    } while (0)
 
 template <int BLOCK, bool PREFETCH>
 static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out) {

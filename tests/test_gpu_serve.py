@@ -1,0 +1,148 @@
+"""GPU parity of served peeling: requests peeled by the resident serving kernel match the oracle bit
+for bit, alone and in a mix with other requests, through ring wrap-around and arena pressure."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import kcoregpu_amd as K
+import oracle as O
+from conftest import closed_form_graphs, csr_from_edges
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def requests(seed, count, median=20000.0, mmax=400000):
+    return [K.Graph.request(seed, i, median_m=median, sigma=1.5, mmax=mmax) for i in range(count)]
+
+
+def expected(gs):
+    return [O.bz(g.neighbors_offset, g.neighbors) for g in gs]
+
+
+def test_serve_stream_matches_oracle(engine):
+    gs = requests(11, 120)
+    ex = expected(gs)
+    say("serve: 120 requests, block 256")
+    with engine.serve(block=256) as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+        done = srv.drain(timeout_s=60)
+        assert len(done) == len(gs)
+        for c in done:
+            i = c["req_id"]
+            assert c["status"] == 0, i
+            assert (c["core"] == ex[i][0]).all(), i
+            assert c["kmax"] == ex[i][1], i
+            assert c["t_submit"] <= c["t_admit"] <= c["t_done"]
+        assert srv.pending() == (0, 0, 0)
+
+
+@pytest.mark.parametrize("block", [512, 1024])
+def test_serve_blocks(engine, block):
+    gs = requests(12, 40)
+    ex = expected(gs)
+    say(f"serve: 40 requests, block {block}")
+    with engine.serve(block=block) as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+        for c in srv.drain(timeout_s=60):
+            assert (c["core"] == ex[c["req_id"]][0]).all(), c["req_id"]
+
+
+def test_serve_alone_equals_mix(engine):
+    """Each request's result is the same when it runs alone on an idle server and when it runs
+    among many others that arrive while it is on the GPU."""
+    probe = requests(13, 12)
+    ex = expected(probe)
+    filler = requests(14, 200)
+    say("serve: 12 probes alone, then inside 200 fillers")
+    with engine.serve() as srv:
+        alone = {}
+        for i, g in enumerate(probe):
+            srv.submit(g, req_id=i)
+            (c,) = srv.drain(timeout_s=30)
+            alone[i] = c["core"]
+        mixed = {}
+        rid = 1000
+        for k, g in enumerate(filler):
+            srv.submit(g, req_id=rid)
+            rid += 1
+            if k % 16 == 5 and k // 16 < len(probe):
+                srv.submit(probe[k // 16], req_id=k // 16)
+            for c in srv.poll():
+                if c["req_id"] < 1000:
+                    mixed[c["req_id"]] = c["core"]
+        for c in srv.drain(timeout_s=60):
+            if c["req_id"] < 1000:
+                mixed[c["req_id"]] = c["core"]
+    assert sorted(mixed) == list(range(len(probe)))
+    for i in range(len(probe)):
+        assert (alone[i] == ex[i][0]).all(), i
+        assert (mixed[i] == alone[i]).all(), i
+
+
+def test_serve_ring_wrap_and_arena_pressure(engine):
+    gs = requests(15, 60, median=30000.0, mmax=200000)
+    ex = expected(gs)
+    need = max(4 * (g.V + 1) + 4 * g.arcs + 12 * g.V + 5 * 256 for g in gs)
+    budget = 3 * need  # a few requests fit at once: the rest wait for memory
+    say(f"serve: ring 4 slots, arena {budget} bytes")
+    with engine.serve(ring_slots=4, hbm_budget=budget, workgroups=8) as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+            q, f, used = srv.pending()
+            assert f <= 4 and used <= budget
+        seen = 0
+        t0 = time.monotonic()
+        while seen < len(gs):
+            for c in srv.poll():
+                assert (c["core"] == ex[c["req_id"]][0]).all(), c["req_id"]
+                seen += 1
+            q, f, used = srv.pending()
+            assert f <= 4 and used <= budget
+            assert time.monotonic() - t0 < 60
+        # a request larger than the whole arena is refused, the server keeps working
+        big = K.Graph.erdos_renyi(200000, 2000000, 3)
+        with pytest.raises(K.KCoreError):
+            srv.submit(big)
+        srv.submit(gs[0], req_id=999)
+        (c,) = srv.drain(timeout_s=30)
+        assert (c["core"] == ex[0][0]).all()
+
+
+def test_serve_closed_forms_and_hub(engine):
+    gs, ex = [], []
+    for name, n, edges, e in closed_form_graphs():
+        off, nbr = csr_from_edges(n, edges)
+        gs.append(K.Graph.from_csr(off, nbr))
+        ex.append(np.array(e, np.uint32))
+    n_leaf = 50000
+    src = [0] * n_leaf + [i for i in range(10) for j in range(i + 1, 10)]
+    dst = list(range(10, 10 + n_leaf)) + [j for i in range(10) for j in range(i + 1, 10)]
+    hub = K.Graph.from_pairs(np.array(src), np.array(dst))
+    gs.append(hub)
+    ex.append(O.bz(hub.neighbors_offset, hub.neighbors)[0])
+    with engine.serve() as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+        for c in srv.drain(timeout_s=30):
+            assert (c["core"] == ex[c["req_id"]]).all(), c["req_id"]
+            assert c["kmax"] == ex[c["req_id"]].max()
+
+
+def test_serve_no_level_skip_same_cores(engine):
+    gs = requests(16, 20)
+    ex = expected(gs)
+    with engine.serve(flags=K.FLAG_NO_LEVEL_SKIP) as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+        for c in srv.drain(timeout_s=60):
+            assert (c["core"] == ex[c["req_id"]][0]).all()
+            assert c["levels"] >= ex[c["req_id"]][1]
