@@ -82,7 +82,10 @@ typedef struct {
 
 enum {
     KC_FLAG_NO_LEVEL_SKIP = 1u << 0, /* scan every level 0..kmax like the reference, even empty ones */
-    KC_FLAG_TRACE = 1u << 8          /* record a device timestamp at every grid barrier */
+    KC_FLAG_TRACE = 1u << 8,         /* record a device timestamp at every grid barrier */
+    KC_FLAG_GRAPH = 1u << 9          /* kc_batch_peel: launch the whole peel (state reset, degree
+                                        copy, persistent kernel) as one HIP graph, captured on first
+                                        use for each (block, flags) and replayed afterwards */
 };
 
 typedef struct {
@@ -149,22 +152,47 @@ kc_status kc_batch_upload(kc_ctx* ctx, const kc_batch_csr* b, kc_batch** out);
 void kc_batch_free(kc_batch* b);
 /* core_out: total_vertices entries (packed like the input), kmax_out: G entries; both may be
  * NULL. stats: levels / subrounds summed over graphs, bytes_alg summed over graphs. Only
- * kc_params.block and .flags (KC_FLAG_NO_LEVEL_SKIP) apply. */
+ * kc_params.block and .flags (KC_FLAG_NO_LEVEL_SKIP, KC_FLAG_GRAPH) apply. With KC_FLAG_GRAPH,
+ * stats.ms_kernel equals stats.ms_peel (the graph is timed as a whole). */
 kc_status kc_batch_peel(kc_ctx* ctx, kc_batch* b, const kc_params* p, uint32_t* core_out,
                         uint32_t* kmax_out, kc_stats* stats);
 /* Per-graph peel order, packed like the vertex arrays (local ids). */
 kc_status kc_batch_download_order(kc_ctx* ctx, const kc_batch* b, uint32_t* order_out);
+void kc_batch_info(const kc_batch* b, uint32_t* G, uint64_t* total_vertices, uint64_t* total_arcs);
+
+/* ---- autotuner (no reference counterpart: the reference hard-codes BLOCK_DIM, BUFF_SIZE and
+ * its warp/block split in inc/common.h) ----
+ * Times real peels of the resident graph (batch) and returns the fastest kc_params: for a graph,
+ * coordinate descent over variant, block, heavy_threshold (the wave/block split), chunk_arcs and
+ * lds_depth starting from the defaults; for a batch, block. Results are identical for every
+ * candidate. cache_path (may be NULL) names a text file of tuned settings keyed by graph shape,
+ * flags and device; a cached key returns at once with *trials = 0. */
+typedef struct {
+    uint32_t reps;          /* timed peels per candidate (median); 0 = 2 */
+    uint32_t flags;         /* KC_FLAG_* every candidate uses */
+    const char* cache_path;
+} kc_tune_cfg;
+kc_status kc_tune_peel(kc_ctx* ctx, kc_graph* g, const kc_tune_cfg* cfg, kc_params* best, double* best_ms,
+                       uint32_t* trials);
+kc_status kc_tune_batch(kc_ctx* ctx, kc_batch* b, const kc_tune_cfg* cfg, kc_params* best, double* best_ms,
+                        uint32_t* trials);
 
 /* ---- served peeling (no reference counterpart) ----
  * A server keeps one persistent kernel resident on the device. Each request is a graph as a host
- * CSR (degrees required); it is admitted, in arrival order, when a ring slot is free and the
- * HBM arena has room for its footprint (offsets + neighbours + three vertex arrays, each rounded
- * to 256 bytes: 8 m + 16 n bytes for n vertices and m undirected edges). Its CSR is copied into
- * its region on a copy stream and published in the ring; an idle workgroup of the kernel takes it
- * at once and peels it alone (requests join and leave the running kernel individually).
- * kc_serve_poll reports finished requests after their core numbers have reached core_out and
- * frees their regions. The CSR arrays and core_out must stay valid until the request's
- * completion is returned. Not thread-safe: one host thread drives a server. */
+ * CSR (offsets and neighbours; kc_csr.degrees is not used, degrees come from the offsets). It is
+ * admitted, in arrival order, when its ring slot is free and the HBM arena has room for its
+ * charge, max(8 m + 16 n bytes, its region) for n vertices and m undirected edges (the region:
+ * offsets, neighbours and two vertex arrays, each rounded up to 256 bytes). Its CSR is copied
+ * into the region on a copy stream and published in the ring; an idle workgroup of the kernel
+ * takes it at once, checks it (offsets monotone from 0 to arcs, neighbour ids < V: a malformed
+ * request completes with KC_ERR_ARG and is never peeled) and peels it alone — requests join and
+ * leave the running kernel individually. When core_out is pinned host memory (kc_host_alloc)
+ * the kernel writes the core numbers there itself; otherwise kc_serve_poll copies them down.
+ * kc_serve_poll reports a request once its core numbers are in core_out and frees its region.
+ * The CSR arrays and core_out must stay valid until the request's completion is returned.
+ * Not thread-safe: one host thread drives a server.
+ * The kernel never outlives the process: idle workgroups exit when the library's heartbeat
+ * thread has been silent for lease_ms, and open servers are stopped at process exit. */
 typedef struct kc_server kc_server;
 typedef struct {
     uint64_t hbm_budget_bytes; /* arena size; 0 = 8 GiB */
@@ -172,17 +200,20 @@ typedef struct {
     uint32_t block;            /* 256, 512 or 1024 threads per workgroup; 0 = 256 */
     uint32_t workgroups;       /* resident serving workgroups; 0 = 2 per CU (256) or 1 per CU */
     uint32_t flags;            /* KC_FLAG_NO_LEVEL_SKIP */
+    uint32_t lease_ms;         /* heartbeat lease of the resident kernel; 0 = 5000 */
 } kc_serve_cfg;
 typedef struct {
     uint64_t req_id;
-    kc_status status;          /* KC_OK, or KC_ERR_NO_PROGRESS when the level loop overran */
+    kc_status status;          /* KC_OK; KC_ERR_ARG for a malformed CSR; KC_ERR_NO_PROGRESS
+                                  when the level loop overran */
     uint32_t kmax, levels, rounds;
     double t_submit, t_admit, t_done; /* kc_serve_clock() seconds */
 } kc_completion;
 kc_status kc_serve_open(kc_ctx* ctx, const kc_serve_cfg* cfg, kc_server** out);
 /* KC_ERR_CAPACITY: V >= 2^30, arcs >= 2^32 or a footprint above the whole arena. */
 kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32_t* core_out);
-/* Writes up to cap completions; *n_out = how many. Also admits waiting requests. */
+/* Writes up to cap completions; *n_out = how many. Also admits waiting requests.
+ * KC_ERR_DEVICE: the kernel has ended (its lease ran out); the server must be closed. */
 kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t* n_out);
 kc_status kc_serve_pending(const kc_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* arena_used);
 /* Stops the kernel after every admitted request has run; KC_ERR_DEVICE if it does not exit. */

@@ -28,6 +28,7 @@ from . import _native as N
 VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4}
 FLAG_NO_LEVEL_SKIP = 1
 FLAG_TRACE = 1 << 8
+FLAG_GRAPH = 1 << 9
 
 KCG_ERRORS = {1: "I/O error", 2: "vertex id or arc count beyond uint32", 3: "bad argument",
               4: "malformed serialized cache"}
@@ -320,16 +321,51 @@ class Engine:
         return srv
 
 
+def _tune(fn, eng, handle, reps, flags, cache_path, single):
+    cfg = N.kc_tune_cfg(reps, flags, os.fsencode(cache_path) if cache_path else None)
+    best, ms, n = N.kc_params(), C.c_double(), C.c_uint32()
+    _check(fn(eng._c, handle, C.byref(cfg), C.byref(best), C.byref(ms), C.byref(n)), fn.__name__)
+    out = {"block": best.block, "flags": best.flags, "ms": ms.value, "trials": n.value}
+    if single:
+        out.update(variant=best.variant, heavy_threshold=best.heavy_threshold,
+                   chunk_arcs=best.chunk_arcs, lds_depth=best.lds_depth)
+    return out
+
+
+class _Pinned:
+    """Owner of one kc_host_alloc block; freed when the last array viewing it is collected."""
+
+    def __init__(self, nbytes: int):
+        self.p = C.c_void_p()
+        _check(N.hip().kc_host_alloc(nbytes, C.byref(self.p)), "kc_host_alloc")
+
+    def __del__(self):
+        if self.p.value:
+            N.hip().kc_host_free(self.p)
+            self.p = C.c_void_p(0)
+
+
+def pinned_empty(n: int, dtype=np.uint32) -> np.ndarray:
+    """A numpy array in pinned host memory (device-accessible; kc_host_alloc)."""
+    dt = np.dtype(dtype)
+    owner = _Pinned(max(n, 1) * dt.itemsize)
+    buf = (C.c_char * (max(n, 1) * dt.itemsize)).from_address(owner.p.value)
+    buf._owner = owner  # the ctypes buffer keeps the allocation alive
+    return np.frombuffer(buf, dtype=dt, count=n)
+
+
 class Server:
     """Served peeling (kc_serve_*): a resident persistent kernel that peels requests as they
     arrive, each in its own region of an HBM arena. submit() never blocks on the GPU; poll()
-    returns finished requests with their core numbers."""
+    returns finished requests with their core numbers. With pinned=True the core numbers land
+    in pinned buffers written by the kernel itself; otherwise they are copied down."""
 
     def __init__(self, eng: "Engine", hbm_budget: int = 0, ring_slots: int = 0, block: int = 0,
-                 workgroups: int = 0, flags: int = 0):
+                 workgroups: int = 0, flags: int = 0, lease_ms: int = 0, pinned: bool = True):
         self.eng = eng
         self._s = C.c_void_p()
-        cfg = N.kc_serve_cfg(hbm_budget, ring_slots, block, workgroups, flags)
+        self.pinned = pinned
+        cfg = N.kc_serve_cfg(hbm_budget, ring_slots, block, workgroups, flags, lease_ms)
         _check(N.hip().kc_serve_open(eng._c, C.byref(cfg), C.byref(self._s)), "kc_serve_open")
         self._live = {}  # req_id -> (graph, core buffer): kept alive until completion
         self._next = 0
@@ -340,9 +376,9 @@ class Server:
         self._next = max(self._next, req_id + 1)
         if req_id in self._live:
             raise ValueError(f"request id {req_id} is still in flight")
-        core = np.empty(g.V, np.uint32)
+        core = pinned_empty(g.V) if self.pinned else np.empty(g.V, np.uint32)
         h = N.kc_csr(g.V, g.arcs, _u32(g.neighbors_offset),
-                     _u32(g.neighbors) if g.arcs else N.u32p(), _u32(g.degrees))
+                     _u32(g.neighbors) if g.arcs else N.u32p(), N.u32p())
         _check(N.hip().kc_serve_submit(self._s, req_id, C.byref(h), _u32(core)), "kc_serve_submit")
         self._live[req_id] = (g, core)
         return req_id
@@ -413,6 +449,10 @@ class DeviceBatch:
         except Exception:
             pass
 
+    def tune(self, reps: int = 2, flags: int = 0, cache_path: str | None = None) -> dict:
+        """Autotuned batch settings (kc_tune_batch): {'block', 'flags', 'ms', 'trials'}."""
+        return _tune(N.hip().kc_tune_batch, self.eng, self._b, reps, flags, cache_path, single=False)
+
     def peel(self, block: int = 0, flags: int = 0, download: bool = True):
         p = N.kc_params(0, block, 0, 0, flags, 0)
         st = N.kc_stats()
@@ -450,7 +490,8 @@ class DeviceGraph:
     def peel(self, variant: str = "auto", block: int = 0, lds_depth: int = 0,
              heavy_threshold: int = 0, flags: int = 0, download: bool = True,
              chunk_arcs: int = 0) -> PeelResult:
-        p = N.kc_params(VARIANTS[variant], block, lds_depth, heavy_threshold, flags, chunk_arcs)
+        v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        p = N.kc_params(v, block, lds_depth, heavy_threshold, flags, chunk_arcs)
         st = N.kc_stats()
         kmax = C.c_uint32()
         core = np.empty(self.V, np.uint32) if download else None
@@ -459,6 +500,11 @@ class DeviceGraph:
                                C.byref(st)), "kc_peel")
         self.kmax = kmax.value
         return PeelResult(core, kmax.value, st.as_dict())
+
+    def tune(self, reps: int = 2, flags: int = 0, cache_path: str | None = None) -> dict:
+        """Autotuned peel settings (kc_tune_peel): a dict of peel() keyword arguments, plus
+        'ms' (median device time of the winner) and 'trials'."""
+        return _tune(N.hip().kc_tune_peel, self.eng, self._g, reps, flags, cache_path, single=True)
 
     def core(self) -> np.ndarray:
         out = np.empty(self.V, np.uint32)

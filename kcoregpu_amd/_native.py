@@ -64,13 +64,17 @@ class kc_stats(C.Structure):
 
 class kc_serve_cfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("ring_slots", C.c_uint32), ("block", C.c_uint32),
-                ("workgroups", C.c_uint32), ("flags", C.c_uint32)]
+                ("workgroups", C.c_uint32), ("flags", C.c_uint32), ("lease_ms", C.c_uint32)]
 
 
 class kc_completion(C.Structure):
     _fields_ = [("req_id", C.c_uint64), ("status", C.c_int), ("kmax", C.c_uint32),
                 ("levels", C.c_uint32), ("rounds", C.c_uint32), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class kc_tune_cfg(C.Structure):
+    _fields_ = [("reps", C.c_uint32), ("flags", C.c_uint32), ("cache_path", C.c_char_p)]
 
 
 def host() -> C.CDLL:
@@ -161,5 +165,11 @@ def hip() -> C.CDLL:
         L.kc_serve_clock.restype = C.c_double
         L.kc_host_alloc.argtypes = [C.c_uint64, pp]
         L.kc_host_free.argtypes = [vp]
+        L.kc_batch_info.argtypes = [vp, u32p, u64p, u64p]
+        L.kc_batch_info.restype = None
+        L.kc_tune_peel.argtypes = [vp, vp, C.POINTER(kc_tune_cfg), C.POINTER(kc_params),
+                                   C.POINTER(C.c_double), u32p]
+        L.kc_tune_batch.argtypes = [vp, vp, C.POINTER(kc_tune_cfg), C.POINTER(kc_params),
+                                    C.POINTER(C.c_double), u32p]
         _hip = L
     return _hip

@@ -13,6 +13,7 @@ struct kc_ctx {
     int cus = 0;
     uint64_t wall_rate_khz = 100000;
     uint64_t wait_limit_ms = 20000;  // one device-side wait; KCORE_WAIT_LIMIT_MS overrides
+    char arch[64] = {0};             // gcnArchName, part of the tuning cache key
 };
 
 inline kc_status hip_status(hipError_t e) {

@@ -167,6 +167,9 @@ kc_status kc_open(int device, kc_ctx** out) {
     hipDeviceProp_t prop;
     KC_TRY(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    for (char* q = c->arch; *q; ++q)
+        if (*q == ' ') *q = '_';
     int rate = 0;
     if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) == hipSuccess && rate > 0)
         c->wall_rate_khz = (uint64_t)rate;
@@ -491,6 +494,12 @@ struct kc_batch {
     uint32_t *d_kmax = nullptr, *d_sched = nullptr, *d_ctrl = nullptr;
     std::vector<uint64_t> vtx_off, arc_off;
     bool peeled = false;
+    struct Captured {
+        uint32_t block, flags;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Captured> graphs;  // captured launch sequences (KC_FLAG_GRAPH)
 };
 
 void kc_batch_free(kc_batch* b) {
@@ -500,6 +509,10 @@ void kc_batch_free(kc_batch* b) {
     for (void* p : {(void*)b->d_vtx, (void*)b->d_arc, (void*)b->d_off, (void*)b->d_nbr, (void*)b->d_deg0,
                     (void*)b->d_core, (void*)b->d_order, (void*)b->d_kmax, (void*)b->d_sched, (void*)b->d_ctrl})
         if (p) (void)hipFree(p);
+    for (auto& cg : b->graphs) {
+        (void)hipGraphExecDestroy(cg.exec);
+        (void)hipGraphDestroy(cg.graph);
+    }
     delete b;
 }
 
@@ -598,20 +611,54 @@ kc_status kc_batch_peel(kc_ctx* c, kc_batch* b, const kc_params* p, uint32_t* co
     a.ctrl = b->d_ctrl;
     a.G = b->G;
     a.flags = prm.flags;
-    KC_TRY(hipEventRecord(c->ev0, c->stream));
-    fill_kernel<uint32_t><<<1, 256, 0, c->stream>>>(b->d_ctrl, 0u, 1024);
-    copy_kernel<<<2048, 256, 0, c->stream>>>(reinterpret_cast<const uint4*>(b->d_deg0),
-                                             reinterpret_cast<uint4*>(b->d_core), (uint32_t)(b->padv / 4));
-    KC_TRY(hipGetLastError());
-    KC_TRY(hipEventRecord(c->evk0, c->stream));
-    kc_status s;
-    switch (block) {
-        case 256: s = launch_batch<256>(c, a); break;
-        case 512: s = launch_batch<512>(c, a); break;
-        default: s = launch_batch<1024>(c, a); break;
+    // the launch sequence: state reset (a fill kernel, not a memset node), degree copy, peel
+    auto enqueue = [&](bool timed) -> kc_status {
+        fill_kernel<uint32_t><<<1, 256, 0, c->stream>>>(b->d_ctrl, 0u, 1024);
+        copy_kernel<<<2048, 256, 0, c->stream>>>(reinterpret_cast<const uint4*>(b->d_deg0),
+                                                 reinterpret_cast<uint4*>(b->d_core), (uint32_t)(b->padv / 4));
+        KC_TRY(hipGetLastError());
+        if (timed) KC_TRY(hipEventRecord(c->evk0, c->stream));
+        kc_status s;
+        switch (block) {
+            case 256: s = launch_batch<256>(c, a); break;
+            case 512: s = launch_batch<512>(c, a); break;
+            default: s = launch_batch<1024>(c, a); break;
+        }
+        if (s != KC_OK) return s;
+        if (timed) KC_TRY(hipEventRecord(c->evk1, c->stream));
+        return KC_OK;
+    };
+    const bool use_graph = (prm.flags & KC_FLAG_GRAPH) != 0;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        for (auto& cg : b->graphs)
+            if (cg.block == block && cg.flags == prm.flags) exec = cg.exec;
+        if (!exec) {
+            kc_batch::Captured cg{block, prm.flags, nullptr, nullptr};
+            KC_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const kc_status s = enqueue(false);
+            const hipError_t ce = hipStreamEndCapture(c->stream, &cg.graph);
+            if (s != KC_OK) {
+                if (cg.graph) (void)hipGraphDestroy(cg.graph);
+                return s;
+            }
+            KC_TRY(ce);
+            const hipError_t ie = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(cg.graph);
+                return hip_status(ie);
+            }
+            b->graphs.push_back(cg);
+            exec = cg.exec;
+        }
     }
-    if (s != KC_OK) return s;
-    KC_TRY(hipEventRecord(c->evk1, c->stream));
+    KC_TRY(hipEventRecord(c->ev0, c->stream));
+    if (use_graph) {
+        KC_TRY(hipGraphLaunch(exec, c->stream));
+    } else {
+        const kc_status s = enqueue(true);
+        if (s != KC_OK) return s;
+    }
     KC_TRY(hipEventRecord(c->ev1, c->stream));
     uint32_t ctl[256];
     KC_TRY(hipMemcpyAsync(ctl, b->d_ctrl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
@@ -619,7 +666,11 @@ kc_status kc_batch_peel(kc_ctx* c, kc_batch* b, const kc_params* p, uint32_t* co
     if (ctl[192]) return KC_ERR_NO_PROGRESS;
     float ms = 0, msk = 0;
     KC_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    KC_TRY(hipEventElapsedTime(&msk, c->evk0, c->evk1));
+    if (use_graph) {
+        msk = ms;
+    } else {
+        KC_TRY(hipEventElapsedTime(&msk, c->evk0, c->evk1));
+    }
     b->peeled = true;
     if (core_out && b->nv)
         KC_TRY(hipMemcpyAsync(core_out, b->d_core, b->nv * 4, hipMemcpyDeviceToHost, c->stream));
@@ -637,6 +688,12 @@ kc_status kc_batch_peel(kc_ctx* c, kc_batch* b, const kc_params* p, uint32_t* co
         stt->bytes_alg = stt->bytes_edge;  // the per-graph scan term is added by the caller
     }
     return KC_OK;
+}
+
+void kc_batch_info(const kc_batch* b, uint32_t* G, uint64_t* total_vertices, uint64_t* total_arcs) {
+    if (G) *G = b->G;
+    if (total_vertices) *total_vertices = b->nv;
+    if (total_arcs) *total_arcs = b->na;
 }
 
 kc_status kc_batch_download_order(kc_ctx* c, const kc_batch* b, uint32_t* order_out) {

@@ -1,23 +1,33 @@
 // serve.hip — host side of served peeling (kc_serve_* in include/kcore.h).
 //
 // A server owns: a device arena of `hbm_budget_bytes` from which each admitted request gets one
-// region (first fit, 256-byte aligned; a request's footprint is its CSR plus three vertex arrays,
-// about 8 m + 16 n bytes for n vertices and m edges); a submission ring in device memory; pinned
-// completion records and a stop flag; two streams — the persistent serving kernel on one, the
-// request uploads, ring publications and result downloads on the other, so copies overlap the
-// peeling of earlier requests.
+// region (first fit, 256-byte aligned: offsets, neighbours, working degrees, peel order); a
+// submission ring in device memory; pinned completion records; a pinned control page (stop flag
+// and heartbeat); two streams — the persistent serving kernel on one, request uploads and ring
+// publications on the other, so copies overlap the peeling of earlier requests.
 //
-// Requests are admitted in arrival order whenever a ring slot is free and the arena has room
-// (a request that does not fit waits; one larger than the whole arena is refused). kc_serve_poll
-// moves completed requests' core numbers to the caller's buffers and frees their regions.
+// Requests are admitted in arrival order whenever their ring slot is free and the arena has room
+// for their charge, max(8 m + 16 n bytes, region size) for n vertices and m edges; a request that
+// does not fit waits, one larger than the whole arena is refused. A request whose core_out is
+// pinned host memory gets its core numbers written there by the kernel itself; otherwise
+// kc_serve_poll copies them down.
+//
+// Lifetime safety: the kernel never outlives its process. A heartbeat thread advances a word in
+// the control page while the server is open; idle kernel workgroups exit when it stops advancing
+// for lease_ms, and an exit handler stops every open server before the HIP runtime shuts down.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "host_internal.h"
@@ -37,14 +47,14 @@ struct Req {
     uint32_t V = 0, arcs = 0;
     const uint32_t* off = nullptr;
     const uint32_t* nbr = nullptr;
-    const uint32_t* deg = nullptr;
     uint32_t* core_out = nullptr;
-    uint64_t base = 0;
+    bool out_pinned = false;
+    uint64_t base = 0, charge = 0;
     ServeRegion r{};
     uint64_t index = 0;
     double t_submit = 0, t_admit = 0;
     hipEvent_t ev = nullptr;
-    int state = 0;  // 0 free, 1 on the device, 2 downloading
+    int state = 0;  // 0 free, 1 on the device, 2 finishing
 };
 
 // first-fit allocator over [0, size)
@@ -93,6 +103,20 @@ hipError_t launch_serve(const ServeArgs& a, uint32_t grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+constexpr uint32_t kStopWord = 32;  // in d_head: device stop flag (stream-ordered)
+constexpr uint32_t kExpired = 48;   // in d_head: set by a workgroup that left on lease expiry
+constexpr uint32_t kHStop = 0;      // in the pinned control page: immediate stop flag
+constexpr uint32_t kHBeat = 16;     // in the pinned control page: heartbeat counter
+
+bool is_pinned(const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeHost && at.hostPointer != nullptr;
+}
+
 }  // namespace
 
 struct kc_server {
@@ -101,40 +125,101 @@ struct kc_server {
     hipStream_t kstream = nullptr, cstream = nullptr;
     uint8_t* arena = nullptr;
     ServeEntry* d_ring = nullptr;
-    uint32_t* d_head = nullptr;  // words 0-1: claim counter (uint64); word 32: stop flag
+    uint32_t* d_head = nullptr;  // words 0-1: claim counter (uint64); kStopWord; kExpired
     ServeDone* h_done = nullptr;
     ServeEntry* h_entry = nullptr;
-    uint32_t* h_stop = nullptr;  // pinned source of the stop flag's value
+    uint32_t* h_ctl = nullptr;  // pinned, coherent: kHStop, kHBeat
+    uint32_t* h_one = nullptr;  // pinned source of the device stop flag's value
     Arena alloc;
     std::deque<Req> queue;
     std::vector<Req> slot;
+    std::vector<uint32_t> fresh;  // slots admitted by one serve_admit call, to publish
     uint64_t next_index = 0;
     uint32_t inflight = 0;
     bool running = false;
+    std::thread hb;  // heartbeat
+    std::mutex hb_m;
+    std::condition_variable hb_cv;
+    bool hb_quit = false;
 };
 
-static constexpr uint32_t kStopWord = 32;
+namespace {
 
-// Stops the serving kernel (the stop flag travels on the copy stream, after every publication)
-// and frees everything. Returns KC_ERR_DEVICE when the kernel did not exit within the wait limit;
-// its memory is then left allocated, since the kernel may still read it.
+std::mutex g_reg_m;
+std::vector<kc_server*> g_servers;
+bool g_atexit = false;
+
+void hb_stop(kc_server* s) {
+    if (!s->hb.joinable()) return;
+    {
+        std::lock_guard<std::mutex> l(s->hb_m);
+        s->hb_quit = true;
+    }
+    s->hb_cv.notify_all();
+    s->hb.join();
+}
+
+// Registered after the HIP runtime initialised, so it runs before the runtime's own teardown:
+// stops every server still open when the process exits.
+void serve_atexit() {
+    std::vector<kc_server*> v;
+    {
+        std::lock_guard<std::mutex> l(g_reg_m);
+        v = g_servers;
+    }
+    for (kc_server* s : v) {
+        if (!s->running) continue;
+        __atomic_store_n(s->h_ctl + kHStop, 1u, __ATOMIC_RELEASE);
+        hb_stop(s);
+        const double t0 = now_s();
+        while (hipStreamQuery(s->kstream) == hipErrorNotReady && now_s() - t0 < 5.0) std::this_thread::yield();
+    }
+}
+
+void reg_add(kc_server* s) {
+    std::lock_guard<std::mutex> l(g_reg_m);
+    if (!g_atexit) {
+        std::atexit(serve_atexit);
+        g_atexit = true;
+    }
+    g_servers.push_back(s);
+}
+
+void reg_del(kc_server* s) {
+    std::lock_guard<std::mutex> l(g_reg_m);
+    g_servers.erase(std::remove(g_servers.begin(), g_servers.end(), s), g_servers.end());
+}
+
+}  // namespace
+
+// Stops the serving kernel (the stop flag travels on the copy stream, after every publication,
+// so every admitted request still runs) and frees everything. Returns KC_ERR_DEVICE when the
+// kernel did not exit within the wait limit; its memory is then left allocated (the immediate
+// stop flag is raised and the heartbeat stopped, so the kernel still ends).
 static kc_status serve_release(kc_server* s) {
     if (s->running) {
-        *s->h_stop = 1;
-        hipError_t e = hipMemcpyAsync(s->d_head + kStopWord, s->h_stop, 4, hipMemcpyHostToDevice, s->cstream);
-        if (e) return hip_status(e);
+        *s->h_one = 1;
+        hipError_t e = hipMemcpyAsync(s->d_head + kStopWord, s->h_one, 4, hipMemcpyHostToDevice, s->cstream);
+        if (e != hipSuccess) __atomic_store_n(s->h_ctl + kHStop, 1u, __ATOMIC_RELEASE);
         const double t0 = now_s();
-        while ((e = hipStreamQuery(s->kstream)) == hipErrorNotReady) {
-            if (now_s() - t0 > s->ctx->wait_limit_ms * 1e-3) return KC_ERR_DEVICE;
+        while (hipStreamQuery(s->kstream) == hipErrorNotReady) {
+            if (now_s() - t0 > s->ctx->wait_limit_ms * 1e-3) {
+                __atomic_store_n(s->h_ctl + kHStop, 1u, __ATOMIC_RELEASE);
+                hb_stop(s);
+                return KC_ERR_DEVICE;
+            }
+            std::this_thread::yield();
         }
         s->running = false;
     }
+    hb_stop(s);
+    reg_del(s);
     if (s->cstream) (void)hipStreamSynchronize(s->cstream);
     for (Req& r : s->slot)
         if (r.ev) (void)hipEventDestroy(r.ev);
     for (void* p : {(void*)s->arena, (void*)s->d_ring, (void*)s->d_head})
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)s->h_done, (void*)s->h_entry, (void*)s->h_stop})
+    for (void* p : {(void*)s->h_done, (void*)s->h_entry, (void*)s->h_ctl, (void*)s->h_one})
         if (p) (void)hipHostFree(p);
     if (s->kstream) (void)hipStreamDestroy(s->kstream);
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
@@ -142,16 +227,18 @@ static kc_status serve_release(kc_server* s) {
     return KC_OK;
 }
 
-// admit queued requests in arrival order while a slot is free and the arena has room
+// admit queued requests in arrival order while their slot is free and the arena has room
 static kc_status serve_admit(kc_server* s) {
+    const uint32_t R = s->cfg.ring_slots;
+    s->fresh.clear();
     while (!s->queue.empty()) {
         // ring slots are reused in index order: the next index waits for its slot's last occupant
-        const uint32_t sl = (uint32_t)(s->next_index % s->cfg.ring_slots);
+        const uint32_t sl = (uint32_t)(s->next_index % R);
         Req& r = s->slot[sl];
         if (r.state != 0) break;
         Req& q = s->queue.front();
         uint64_t base;
-        if (!s->alloc.alloc(q.r.bytes, &base)) break;
+        if (!s->alloc.alloc(q.charge, &base)) break;
         hipEvent_t ev = r.ev;
         r = q;
         r.ev = ev;
@@ -162,19 +249,28 @@ static kc_status serve_admit(kc_server* s) {
         r.state = 1;
         s->inflight++;
         uint8_t* dst = s->arena + base;
-        hipStream_t st = s->cstream;
-        KC_TRY(hipMemcpyAsync(dst + r.r.off, r.off, ((size_t)r.V + 1) * 4, hipMemcpyHostToDevice, st));
-        if (r.arcs) KC_TRY(hipMemcpyAsync(dst + r.r.nbr, r.nbr, (size_t)r.arcs * 4, hipMemcpyHostToDevice, st));
-        KC_TRY(hipMemcpyAsync(dst + r.r.deg0, r.deg, (size_t)r.V * 4, hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(dst + r.r.off, r.off, ((size_t)r.V + 1) * 4, hipMemcpyHostToDevice, s->cstream));
+        if (r.arcs)
+            KC_TRY(hipMemcpyAsync(dst + r.r.nbr, r.nbr, (size_t)r.arcs * 4, hipMemcpyHostToDevice, s->cstream));
         ServeEntry* e = s->h_entry + sl;
         e->base = base;
         e->V = r.V;
         e->arcs = r.arcs;
         e->flags = s->cfg.flags;
         e->pad = 0;
+        e->out = r.out_pinned ? r.core_out : nullptr;
+        e->pad2 = 0;
         e->seq = r.index + 1;
-        // the entry goes last on the same stream: the kernel sees it only after the data
-        KC_TRY(hipMemcpyAsync(s->d_ring + sl, e, sizeof(ServeEntry), hipMemcpyHostToDevice, st));
+        s->fresh.push_back(sl);
+    }
+    // Entries go after all data on the same stream, so the kernel sees an entry only after its
+    // request's data; consecutive slots are published by one copy.
+    for (size_t i = 0; i < s->fresh.size();) {
+        size_t j = i + 1;
+        while (j < s->fresh.size() && s->fresh[j] == s->fresh[j - 1] + 1) ++j;
+        KC_TRY(hipMemcpyAsync(s->d_ring + s->fresh[i], s->h_entry + s->fresh[i], (j - i) * sizeof(ServeEntry),
+                              hipMemcpyHostToDevice, s->cstream));
+        i = j;
     }
     return KC_OK;
 }
@@ -193,6 +289,7 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 8ull << 30;
     if (!s->cfg.ring_slots) s->cfg.ring_slots = 1024;
     if (!s->cfg.block) s->cfg.block = 256;
+    if (!s->cfg.lease_ms) s->cfg.lease_ms = 5000;
     if (s->cfg.block != 256 && s->cfg.block != 512 && s->cfg.block != 1024) {
         delete s;
         return KC_ERR_ARG;
@@ -200,6 +297,7 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     // leave room on every CU for the runtime's own copy kernels next to the resident server
     if (!s->cfg.workgroups) s->cfg.workgroups = (uint32_t)c->cus * (s->cfg.block == 256 ? 2 : 1);
     const uint32_t R = s->cfg.ring_slots;
+    reg_add(s);
     auto fail = [&](hipError_t e) {
         (void)serve_release(s);
         return hip_status(e);
@@ -210,13 +308,15 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     if ((e = hipMalloc(&s->arena, s->cfg.hbm_budget_bytes))) return fail(e);
     if ((e = hipMalloc(&s->d_ring, (size_t)R * sizeof(ServeEntry)))) return fail(e);
     if ((e = hipMalloc(&s->d_head, 256))) return fail(e);
-    // completion records are written by the kernel and polled by the host: coherent memory
+    // completion records and the control page are shared live with the kernel: coherent memory
     if ((e = hipHostMalloc(&s->h_done, (size_t)R * sizeof(ServeDone), hipHostMallocCoherent))) return fail(e);
+    if ((e = hipHostMalloc(&s->h_ctl, 256, hipHostMallocCoherent))) return fail(e);
     if ((e = hipHostMalloc(&s->h_entry, (size_t)R * sizeof(ServeEntry), hipHostMallocDefault))) return fail(e);
-    if ((e = hipHostMalloc(&s->h_stop, 256, hipHostMallocDefault))) return fail(e);
+    if ((e = hipHostMalloc(&s->h_one, 256, hipHostMallocDefault))) return fail(e);
     memset(s->h_done, 0, (size_t)R * sizeof(ServeDone));
     memset(s->h_entry, 0, (size_t)R * sizeof(ServeEntry));
-    *s->h_stop = 0;
+    memset(s->h_ctl, 0, 256);
+    *s->h_one = 0;
     if ((e = hipMemsetAsync(s->d_ring, 0, (size_t)R * sizeof(ServeEntry), s->kstream))) return fail(e);
     if ((e = hipMemsetAsync(s->d_head, 0, 256, s->kstream))) return fail(e);
     // the ring must be clear before the copy stream can publish into it
@@ -225,12 +325,25 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     for (Req& r : s->slot)
         if ((e = hipEventCreateWithFlags(&r.ev, hipEventDisableTiming))) return fail(e);
     s->alloc.init(s->cfg.hbm_budget_bytes);
+    // heartbeat: advances at least 20 times per lease
+    const uint32_t period_ms = std::max<uint32_t>(1, std::min<uint32_t>(50, s->cfg.lease_ms / 20));
+    s->hb = std::thread([s, period_ms] {
+        std::unique_lock<std::mutex> l(s->hb_m);
+        while (!s->hb_quit) {
+            __atomic_fetch_add(s->h_ctl + kHBeat, 1u, __ATOMIC_RELEASE);
+            s->hb_cv.wait_for(l, std::chrono::milliseconds(period_ms));
+        }
+    });
     ServeArgs a{};
     a.arena = s->arena;
     a.ring = s->d_ring;
     a.done = s->h_done;
     a.head = reinterpret_cast<uint64_t*>(s->d_head);
     a.stop = s->d_head + kStopWord;
+    a.hstop = s->h_ctl + kHStop;
+    a.hbeat = s->h_ctl + kHBeat;
+    a.expired = s->d_head + kExpired;
+    a.lease_ticks = (uint64_t)s->cfg.lease_ms * c->wall_rate_khz;
     a.slots = R;
     switch (s->cfg.block) {
         case 256: e = launch_serve<256>(a, s->cfg.workgroups, s->kstream); break;
@@ -244,19 +357,21 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
 }
 
 kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32_t* core_out) {
-    if (!s || !g || !g->offsets || (g->arcs && !g->neighbors) || !g->degrees) return KC_ERR_ARG;
+    if (!s || !g || !g->offsets || (g->arcs && !g->neighbors)) return KC_ERR_ARG;
     if (g->V == 0 || g->V >= (1u << 30) || g->arcs >= 0xFFFFFFFFull) return KC_ERR_CAPACITY;
     if (g->offsets[0] != 0 || g->offsets[g->V] != g->arcs) return KC_ERR_ARG;
+    if (!s->running) return KC_ERR_DEVICE;
     Req q;
     q.id = req_id;
     q.V = g->V;
     q.arcs = (uint32_t)g->arcs;
     q.off = g->offsets;
     q.nbr = g->neighbors;
-    q.deg = g->degrees;
     q.core_out = core_out;
+    q.out_pinned = core_out && is_pinned(core_out);
     q.r = serve_region(q.V, q.arcs);
-    if (q.r.bytes > s->cfg.hbm_budget_bytes) return KC_ERR_CAPACITY;
+    q.charge = std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
+    if (q.charge > s->cfg.hbm_budget_bytes) return KC_ERR_CAPACITY;
     q.t_submit = now_s();
     s->queue.push_back(q);
     return serve_admit(s);
@@ -264,39 +379,48 @@ kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32
 
 kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t* n_out) {
     if (!s || !n_out || (cap && !out)) return KC_ERR_ARG;
+    *n_out = 0;
+    if (!s->running) return KC_ERR_DEVICE;
     uint32_t n = 0;
     for (uint32_t sl = 0; sl < s->cfg.ring_slots && s->inflight; ++sl) {
         Req& r = s->slot[sl];
         if (r.state == 1) {
             const uint64_t seq = __atomic_load_n(&s->h_done[sl].seq, __ATOMIC_ACQUIRE);
             if (seq != r.index + 1) continue;
-            if (r.core_out) {
+            if (r.core_out && !r.out_pinned) {
                 KC_TRY(hipMemcpyAsync(r.core_out, s->arena + r.base + r.r.core, (size_t)r.V * 4,
                                       hipMemcpyDeviceToHost, s->cstream));
+                KC_TRY(hipEventRecord(r.ev, s->cstream));
             }
-            KC_TRY(hipEventRecord(r.ev, s->cstream));
             r.state = 2;
         }
         if (r.state == 2 && n < cap) {
-            const hipError_t q = hipEventQuery(r.ev);
-            if (q == hipErrorNotReady) continue;
-            KC_TRY(q);
+            if (r.core_out && !r.out_pinned) {
+                const hipError_t q = hipEventQuery(r.ev);
+                if (q == hipErrorNotReady) continue;
+                KC_TRY(q);
+            }
             kc_completion& c = out[n++];
             const ServeDone& d = s->h_done[sl];
             c.req_id = r.id;
             c.kmax = d.kmax;
             c.levels = d.levels;
             c.rounds = d.rounds;
-            c.status = d.err ? KC_ERR_NO_PROGRESS : KC_OK;
+            c.status = d.err == SERVE_OK ? KC_OK : (d.err == SERVE_BAD_CSR ? KC_ERR_ARG : KC_ERR_NO_PROGRESS);
             c.t_submit = r.t_submit;
             c.t_admit = r.t_admit;
             c.t_done = now_s();
-            s->alloc.release(r.base, r.r.bytes);
+            s->alloc.release(r.base, r.charge);
             r.state = 0;
             s->inflight--;
         }
     }
     *n_out = n;
+    // a kernel that ended while the server is open (its lease ran out) is reported, not waited on
+    if (n == 0 && s->inflight && hipStreamQuery(s->kstream) == hipSuccess) {
+        s->running = false;
+        return KC_ERR_DEVICE;
+    }
     return serve_admit(s);
 }
 

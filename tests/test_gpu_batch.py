@@ -100,3 +100,36 @@ def test_malformed_inputs_rejected(engine):
     hb.vtx_off = vo.ctypes.data_as(N.u64p)
     hb.neighbors = b.neighbors.ctypes.data_as(N.u32p)
     assert N.hip().kc_batch_upload(engine._c, C.byref(hb), C.byref(out)) == 1
+
+
+def test_batch_graph_replays_match_eager(engine):
+    """The captured launch sequence replays correctly many times, with other eager work between
+    the replays (no memset node is captured; the state reset is a fill kernel)."""
+    b = K.Batch.generate(64, 21, nmin=500, nmax=20000)
+    other = K.Batch.generate(16, 22, nmin=500, nmax=5000)
+    db, do = engine.upload_batch(b), engine.upload_batch(other)
+    core, kmax, _ = db.peel()
+    ocore, okmax, _ = do.peel()
+    for block in (512, 1024):
+        for rep in range(4):
+            c2, k2, st = db.peel(block=block, flags=K.FLAG_GRAPH)
+            assert (c2 == core).all() and (k2 == kmax).all(), (block, rep)
+            assert st["ms_kernel"] == st["ms_peel"]
+            c3, k3, _ = do.peel()
+            assert (c3 == ocore).all() and (k3 == okmax).all(), (block, rep)
+    check_batch(b, core, kmax, db.order())
+    db.free()
+    do.free()
+
+
+def test_batch_tuner_and_cache(engine, tmp_path):
+    b = K.Batch.generate(48, 23, nmin=500, nmax=20000)
+    db = engine.upload_batch(b)
+    cache = str(tmp_path / "tune.txt")
+    t = db.tune(cache_path=cache)
+    assert t["block"] in (256, 512, 1024) and t["trials"] == 3 and t["ms"] > 0
+    t2 = db.tune(cache_path=cache)
+    assert t2["trials"] == 0 and t2["block"] == t["block"]
+    core, kmax, _ = db.peel(block=t["block"])
+    check_batch(b, core, kmax)
+    db.free()

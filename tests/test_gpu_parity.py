@@ -172,3 +172,19 @@ def test_rmat24_full_size(engine):
     r2 = dg.peel("auto", flags=K.FLAG_NO_LEVEL_SKIP)
     assert (r2.core == core).all() and r2.stats["levels"] == kmax + 1
     dg.free()
+
+
+def test_tuner_picks_valid_params_and_caches(engine, tmp_path):
+    g = K.Graph.chung_lu(100000, 31)
+    ref, km = O.bz(g.neighbors_offset, g.neighbors)
+    dg = engine.upload(g)
+    cache = str(tmp_path / "tune.txt")
+    t = dg.tune(cache_path=cache)
+    assert t["trials"] >= 10 and t["block"] in (256, 512, 1024)
+    t2 = dg.tune(cache_path=cache)
+    assert t2["trials"] == 0
+    assert {k: t2[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")} == \
+        {k: t[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")}
+    r = dg.peel(**{k: t[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")})
+    assert (r.core == ref).all() and r.kmax == km
+    dg.free()

@@ -146,3 +146,45 @@ def test_serve_no_level_skip_same_cores(engine):
         for c in srv.drain(timeout_s=60):
             assert (c["core"] == ex[c["req_id"]][0]).all()
             assert c["levels"] >= ex[c["req_id"]][1]
+
+
+def test_serve_copy_path_and_malformed_request(engine):
+    """Pageable output buffers take the copy-down path; a malformed request completes with
+    KC_ERR_ARG without being peeled, and the server keeps serving."""
+    gs = requests(17, 10)
+    ex = expected(gs)
+    bad = K.Graph.erdos_renyi(2000, 8000, 5)
+    with engine.serve(pinned=False) as srv:
+        for i, g in enumerate(gs[:5]):
+            srv.submit(g, req_id=i)
+        # neighbour id == V in one adjacency list: rejected by the kernel's check
+        nbr = bad.neighbors.copy()
+        nbr[100] = bad.V
+        import ctypes as C
+        from kcoregpu_amd import _native as N
+        out = np.empty(bad.V, np.uint32)
+        h = N.kc_csr(bad.V, bad.arcs, bad.neighbors_offset.ctypes.data_as(N.u32p),
+                     nbr.ctypes.data_as(N.u32p), N.u32p())
+        assert N.hip().kc_serve_submit(srv._s, 777, C.byref(h), out.ctypes.data_as(N.u32p)) == 0
+        srv._live[777] = (bad, out)  # keeps the buffers alive until the completion
+        for i, g in enumerate(gs[5:], start=5):
+            srv.submit(g, req_id=i)
+        got = {c["req_id"]: c for c in srv.drain(timeout_s=60)}
+    assert got[777]["status"] == 1  # KC_ERR_ARG
+    for i in range(len(gs)):
+        assert got[i]["status"] == 0
+        assert (got[i]["core"] == ex[i][0]).all(), i
+
+
+def test_serve_survives_idle_longer_than_lease(engine):
+    """The heartbeat keeps an idle server's kernel alive past its lease."""
+    g = requests(18, 1)[0]
+    ref = O.bz(g.neighbors_offset, g.neighbors)[0]
+    with engine.serve(lease_ms=200, workgroups=16) as srv:
+        srv.submit(g, req_id=0)
+        (c,) = srv.drain(timeout_s=30)
+        assert (c["core"] == ref).all()
+        time.sleep(1.0)  # five leases without a request
+        srv.submit(g, req_id=1)
+        (c,) = srv.drain(timeout_s=30)
+        assert (c["core"] == ref).all()
