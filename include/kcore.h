@@ -67,7 +67,10 @@ enum {
     KC_VARIANT_WAVE = 1,     /* one persistent kernel; each wave drains global frontier tickets */
     KC_VARIANT_LDS = 2,      /* + workgroup LDS frontier buffer for cascades, spilling to global */
     KC_VARIANT_PREFETCH = 3, /* + next-batch adjacency prefetch while the current batch runs */
-    KC_VARIANT_STEPWISE = 4  /* reference-shaped: select and process kernels launched per level */
+    KC_VARIANT_STEPWISE = 4, /* reference-shaped: select and process kernels launched per level */
+    KC_VARIANT_ASYNC = 5     /* one persistent kernel; two grid barriers per level: the level's
+                                cascade runs from a shared work queue with no barrier inside it,
+                                and scans read a compacted list once few vertices are left */
 };
 
 typedef struct {
@@ -78,6 +81,10 @@ typedef struct {
                                  by many waves; 0 = default, UINT32_MAX = never split */
     uint32_t flags;           /* KC_FLAG_* */
     uint32_t chunk_arcs;      /* arcs per chunk; 0 = default */
+    uint32_t compact_ratio;   /* ASYNC: a scan compacts the alive list when unpeeled vertices *
+                                 compact_ratio < entries it reads; 0 = default (4), UINT32_MAX =
+                                 never compact */
+    uint32_t poll_max;        /* ASYNC: longest idle poll sleep step, 0..3; 0 = default (2) */
 } kc_params;
 
 enum {

@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _native as N
 
-VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4}
+VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4, "async": 5}
 FLAG_NO_LEVEL_SKIP = 1
 FLAG_TRACE = 1 << 8
 FLAG_GRAPH = 1 << 9
@@ -489,9 +489,9 @@ class DeviceGraph:
 
     def peel(self, variant: str = "auto", block: int = 0, lds_depth: int = 0,
              heavy_threshold: int = 0, flags: int = 0, download: bool = True,
-             chunk_arcs: int = 0) -> PeelResult:
+             chunk_arcs: int = 0, compact_ratio: int = 0, poll_max: int = 0) -> PeelResult:
         v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
-        p = N.kc_params(v, block, lds_depth, heavy_threshold, flags, chunk_arcs)
+        p = N.kc_params(v, block, lds_depth, heavy_threshold, flags, chunk_arcs, compact_ratio, poll_max)
         st = N.kc_stats()
         kmax = C.c_uint32()
         core = np.empty(self.V, np.uint32) if download else None

@@ -47,7 +47,8 @@ class kc_batch_csr(C.Structure):
 
 class kc_params(C.Structure):
     _fields_ = [("variant", C.c_int), ("block", C.c_uint32), ("lds_depth", C.c_uint32),
-                ("heavy_threshold", C.c_uint32), ("flags", C.c_uint32), ("chunk_arcs", C.c_uint32)]
+                ("heavy_threshold", C.c_uint32), ("flags", C.c_uint32), ("chunk_arcs", C.c_uint32),
+                ("compact_ratio", C.c_uint32), ("poll_max", C.c_uint32)]
 
 
 class kc_stats(C.Structure):

@@ -12,6 +12,7 @@
 #include "../../include/kcore.h"
 #include "batch_kernel.h"
 #include "host_internal.h"
+#include "peel_async.h"
 #include "peel_kernel.h"
 
 using namespace kcdev;
@@ -106,16 +107,22 @@ struct kc_graph {
     uint64_t* d_trace = nullptr;
     uint32_t* d_dbg = nullptr;
     uint32_t* d_ctrl = nullptr;
+    uint32_t* d_alive = nullptr;  // two alive lists of padV entries (ASYNC), allocated on first use
+    uint32_t chunk_hwm = 0;       // chunk slots written by earlier peels (re-zeroed before ASYNC)
     uint32_t last_grid = 0;
     uint32_t last_kmax = 0;
     uint32_t last_ntrace = 0;
     bool peeled = false;
 };
 
-template <int BLOCK, bool PREFETCH>
+template <int BLOCK, bool PREFETCH, bool ASYNC>
 static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out) {
     int per_cu = 0;
-    KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_persistent<BLOCK, PREFETCH>, BLOCK, 0));
+    if (ASYNC) {
+        KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_async<BLOCK, PREFETCH>, BLOCK, 0));
+    } else {
+        KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_persistent<BLOCK, PREFETCH>, BLOCK, 0));
+    }
     if (per_cu < 1) return KC_ERR_HIP;
     uint32_t grid = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)c->cus, kMaxGrid);
     *grid_out = grid;
@@ -123,7 +130,11 @@ static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out
     if (a.lq_depth > (uint32_t)WgState<BLOCK>::LQ) a.lq_depth = WgState<BLOCK>::LQ;
     // A plain launch: the kernel's elastic start (register_workgroup) only lets workgroups
     // that are resident together take part, so it needs no co-residency guarantee.
-    peel_persistent<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    if (ASYNC) {
+        peel_async<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    } else {
+        peel_persistent<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    }
     KC_TRY(hipGetLastError());
     return KC_OK;
 }
@@ -202,7 +213,8 @@ void kc_graph_free(kc_graph* g) {
     (void)hipStreamSynchronize(g->ctx->stream);
     for (void* p : {(void*)g->d_off, (void*)g->d_nbr, (void*)g->d_deg0, (void*)g->d_core,
                     (void*)g->d_order, (void*)g->d_work, (void*)g->d_chunks, (void*)g->d_shell,
-                    (void*)g->d_wgmin, (void*)g->d_trace, (void*)g->d_dbg, (void*)g->d_ctrl})
+                    (void*)g->d_wgmin, (void*)g->d_trace, (void*)g->d_dbg, (void*)g->d_ctrl,
+                    (void*)g->d_alive})
         if (p) (void)hipFree(p);
     delete g;
 }
@@ -250,7 +262,7 @@ kc_status kc_graph_upload(kc_ctx* c, const kc_csr* h, kc_graph** out) {
     KC_ALLOC(g->d_wgmin, (size_t)kMaxGrid * 4);
     KC_ALLOC(g->d_trace, (size_t)kTraceCap * 16);
     KC_ALLOC(g->d_dbg, (size_t)kMaxGrid * 8 * 4);
-    KC_ALLOC(g->d_ctrl, (size_t)C_COUNT * kStride * 4);
+    KC_ALLOC(g->d_ctrl, (size_t)A_COUNT * kStride * 4);
 #undef KC_ALLOC
     if ((e = hipMemcpyAsync(g->d_off, h->offsets, ((size_t)h->V + 1) * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
     if (h->arcs && (e = hipMemcpyAsync(g->d_nbr, h->neighbors, h->arcs * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
@@ -260,6 +272,8 @@ kc_status kc_graph_upload(kc_ctx* c, const kc_csr* h, kc_graph** out) {
     } else {
         degrees_from_offsets<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_deg0, g->V, g->padV);
     }
+    // chunk slots start empty (0): the ASYNC kernel waits on a claimed slot until it is written
+    fill_kernel<uint64_t><<<1024, 256, 0, c->stream>>>(g->d_chunks, 0ull, g->chunk_cap);
     // reject a malformed CSR before any kernel indexes with it
     fill_kernel<uint32_t><<<1, 64, 0, c->stream>>>(g->d_ctrl, 0u, 1);
     validate_csr<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_nbr, g->V, g->arcs, g->d_ctrl);
@@ -288,7 +302,10 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
     kc_params prm{};
     if (p) prm = *p;
     int variant = prm.variant ? prm.variant : KC_VARIANT_LDS;
-    if (variant < KC_VARIANT_WAVE || variant > KC_VARIANT_PREFETCH) return KC_ERR_ARG;
+    if (variant != KC_VARIANT_ASYNC && (variant < KC_VARIANT_WAVE || variant > KC_VARIANT_PREFETCH))
+        return KC_ERR_ARG;
+    const bool async = variant == KC_VARIANT_ASYNC;
+    if (async && !g->d_alive) KC_TRY(hipMalloc(&g->d_alive, (size_t)g->padV * 8));
     const uint32_t block = prm.block ? prm.block : 512;
     if (block != 256 && block != 512 && block != 1024) return KC_ERR_ARG;
     uint32_t heavy = prm.heavy_threshold ? prm.heavy_threshold : 64;
@@ -322,9 +339,20 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
     a.trace_cap = kTraceCap;
     a.time_limit = c->wall_rate_khz * c->wait_limit_ms;
     a.reg_window = c->wall_rate_khz * 30 / 1000;  // 30 us without a new check-in
+    a.alive[0] = g->d_alive;
+    a.alive[1] = g->d_alive ? g->d_alive + g->padV : nullptr;
+    a.async_mode = async ? 1u : 0u;
+    a.compact_ratio = prm.compact_ratio == 0xFFFFFFFFu ? 0u : (prm.compact_ratio ? prm.compact_ratio : 4u);
+    a.poll_max = prm.poll_max ? std::min<uint32_t>(prm.poll_max, 3u) : 2u;
 
     KC_TRY(hipEventRecord(c->ev0, c->stream));
-    fill_kernel<uint32_t><<<256, 256, 0, c->stream>>>(g->d_ctrl, 0u, (uint64_t)C_COUNT * kStride);
+    fill_kernel<uint32_t><<<256, 256, 0, c->stream>>>(g->d_ctrl, 0u, (uint64_t)A_COUNT * kStride);
+    if (async) {
+        // queue slots start as "not yet written": order[] ~0, chunk slots 0
+        fill_kernel<uint32_t><<<1024, 256, 0, c->stream>>>(g->d_order, kSent, g->V);
+        if (g->chunk_hwm)
+            fill_kernel<uint64_t><<<1024, 256, 0, c->stream>>>(g->d_chunks, 0ull, g->chunk_hwm);
+    }
     fill_kernel<uint32_t><<<64, 256, 0, c->stream>>>(g->d_shell, 0u, (uint64_t)g->dmax + 3);
     fill_kernel<uint32_t><<<64, 256, 0, c->stream>>>(g->d_dbg, 0u, (uint64_t)kMaxGrid * 8);
     KC_TRY(hipGetLastError());
@@ -335,16 +363,24 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
                                              reinterpret_cast<uint4*>(g->d_core), g->padV / 4);
     KC_TRY(hipGetLastError());
     const bool pf = variant == KC_VARIANT_PREFETCH;
-    switch (block) {
-        case 256: s = pf ? launch_peel<256, true>(c, a, &grid) : launch_peel<256, false>(c, a, &grid); break;
-        case 512: s = pf ? launch_peel<512, true>(c, a, &grid) : launch_peel<512, false>(c, a, &grid); break;
-        default: s = pf ? launch_peel<1024, true>(c, a, &grid) : launch_peel<1024, false>(c, a, &grid); break;
+    if (async) {
+        switch (block) {
+            case 256: s = launch_peel<256, false, true>(c, a, &grid); break;
+            case 512: s = launch_peel<512, false, true>(c, a, &grid); break;
+            default: s = launch_peel<1024, false, true>(c, a, &grid); break;
+        }
+    } else {
+        switch (block) {
+            case 256: s = pf ? launch_peel<256, true, false>(c, a, &grid) : launch_peel<256, false, false>(c, a, &grid); break;
+            case 512: s = pf ? launch_peel<512, true, false>(c, a, &grid) : launch_peel<512, false, false>(c, a, &grid); break;
+            default: s = pf ? launch_peel<1024, true, false>(c, a, &grid) : launch_peel<1024, false, false>(c, a, &grid); break;
+        }
     }
     if (s != KC_OK) return s;
     g->last_grid = grid;
     KC_TRY(hipEventRecord(c->evk1, c->stream));
     KC_TRY(hipEventRecord(c->ev1, c->stream));
-    uint32_t res[R_WORDS];
+    uint32_t res[R_WORDS + 4];
     KC_TRY(hipMemcpyAsync(res, g->d_ctrl + (size_t)C_RESULT * kStride, sizeof(res), hipMemcpyDeviceToHost, c->stream));
     KC_TRY(hipStreamSynchronize(c->stream));
     float ms = 0, msk = 0;
@@ -355,6 +391,7 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
     if (err == 2) return KC_ERR_NO_PROGRESS;
     if (res[R_NPART] == 0) return KC_ERR_HIP;  // the kernel never ran
     g->last_kmax = res[R_KMAX];
+    g->chunk_hwm = std::max(g->chunk_hwm, std::min(res[R_WORDS + 3], g->chunk_cap));
     g->last_ntrace = std::min(res[R_NTRACE], kTraceCap);
     g->peeled = true;
     double d2h = 0;
@@ -376,8 +413,14 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
         st->kmax = g->last_kmax;
         st->levels = res[R_LEVELS];
         st->subrounds = res[R_SUBROUNDS];
-        st->scanned_entries = (uint64_t)st->levels * g->padV;
-        st->bytes_scan = 4 * st->scanned_entries;
+        if (async) {
+            // dense entries (4 B), list entries (4 B id + 4 B gathered degree), list writes (4 B)
+            st->scanned_entries = (uint64_t)res[R_WORDS] + res[R_WORDS + 1];
+            st->bytes_scan = 4ull * res[R_WORDS] + 8ull * res[R_WORDS + 1] + 4ull * res[R_WORDS + 2];
+        } else {
+            st->scanned_entries = (uint64_t)st->levels * g->padV;
+            st->bytes_scan = 4 * st->scanned_entries;
+        }
         st->bytes_edge = 16ull * g->V + 20ull * (g->arcs / 2);
         st->bytes_alg = st->bytes_scan + st->bytes_edge;
         st->grid = grid;

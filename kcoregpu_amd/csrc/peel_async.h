@@ -1,0 +1,450 @@
+// peel_async.h — the cascade-queue single-graph peeling kernel (KC_VARIANT_ASYNC).
+//
+// Same method as peel_persistent (the reference's level-synchronous peel, src/ours.cc:2-160) and
+// the same per-arc rule, but a level has exactly two grid barriers, whatever its cascade depth:
+//
+//  * scan: every workgroup takes a static share of the vertices still to be peeled and records
+//    those with degree == L in order[] (cutting heavy ones into arc chunks). Once few vertices
+//    are left unpeeled (unpeeled * compact_ratio < entries a scan reads) a scan also writes the
+//    vertices it finds with degree > L to a fresh alive list, and later scans read that list
+//    (ids, then their degrees) instead of the whole degree array.
+//  * barrier, then processing with no barrier inside the level: order[] and the chunk list are
+//    work queues. A workgroup claims up to BLOCK entries of order[] or one chunk per wave with a
+//    compare-and-swap on the queue head, processes them (decrements, undo, crossings), and its
+//    crossings are appended to order[] (and chunked when heavy) where any workgroup can claim
+//    them at once. Slots are pre-filled with a sentinel (order[]: ~0, chunks: 0), so a claimed
+//    slot whose writer reserved it but has not stored it yet is simply waited for.
+//  * termination: VDONE / CDONE count processed order[] entries / chunks over the whole peel.
+//    After a workgroup's appends are complete it adds what it processed; whoever then reads
+//    VDONE == OTAIL and CDONE == CTAIL (done counters read first, tails after) knows that
+//    nothing is in flight and nothing more can be appended, and raises the level's done flag
+//    (one copy per XCD group, tagged with the level's generation).
+//  * barrier: the last workgroup computes the next level (smallest degree left above L), the
+//    shell offsets and whether the next scan compacts.
+// Cross-workgroup data uses the same device-scope (SC1) accesses as peel_persistent; there are
+// no cache-wide fences.
+#pragma once
+#include "peel_kernel.h"
+
+namespace kcdev {
+
+enum : uint32_t {
+    A_VHEAD = C_COUNT,  // next order[] entry to claim
+    A_VDONE,            // order[] entries processed
+    A_CHEAD,            // next chunk to claim
+    A_CDONE,            // chunks processed
+    A_MODE,             // 0: scans read the dense degree array, 1: an alive list
+    A_LIST,             // current alive list (0/1)
+    A_LEN,              // its length
+    A_COMPACT,          // 1: the next scan writes a new alive list into A_DST
+    A_DST,
+    A_ALEN0,            // append counters of the two alive lists
+    A_ALEN1,
+    A_NEXT,             // next level
+    A_PHASE,            // PH_NEXT_LEVEL / PH_DONE
+    A_GEN,              // generation of the current processing phase
+    A_SCAN_DENSE,       // scan accounting: dense entries read
+    A_SCAN_LIST,        // list entries read (id + gathered degree)
+    A_SURV,             // alive-list entries written
+    A_SVBEG,            // the level's scan frontier: order[] range and chunk range, dealt
+    A_SVEND,            // statically over every wave before the queues open
+    A_SCBEG,
+    A_SCEND,
+    A_LDONE,            // kXcds level-done flags
+    A_COUNT = A_LDONE + kXcds
+};
+
+template <int BLOCK>
+struct WgStateA : WgState<BLOCK> {
+    uint32_t surv[BLOCK / 64][kStageCap];
+    uint32_t kind, beg, n;
+    uint32_t vcount, ccount;
+};
+
+__device__ __forceinline__ void order_point() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    drain_memory();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ bool cas_rlx(uint32_t* p, uint32_t expect, uint32_t want) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, KC_AGENT);
+}
+
+// Survivors (degree > L) staged per wave and appended to the alive list being built.
+template <int BLOCK>
+__device__ void flush_surv(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t& n, uint32_t dst) {
+    if (n == 0) return;
+    const uint32_t lane = lane_id();
+    const uint32_t* st = S.surv[threadIdx.x / 64];
+    const uint32_t b = wave_reserve(cw(a, A_ALEN0 + dst), n);
+    for (uint32_t i = lane; i < n; i += 64) st_rlx(a.alive[dst] + b + i, st[i]);
+    n = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void surv_push(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t& n, bool pred, uint32_t v,
+                                          uint32_t dst) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t cnt = __popcll(m);
+    if (n + cnt > kStageCap) flush_surv(a, S, n, dst);
+    if (pred) S.surv[threadIdx.x / 64][n + rank_below(m)] = v;
+    n += cnt;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Scan of the dense degree array (16-byte SC1 loads, four in flight per lane).
+template <int BLOCK>
+__device__ void scan_dense(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, bool compact,
+                           uint32_t dst) {
+    constexpr uint32_t WPB = BLOCK / 64;
+    constexpr uint32_t U = 4;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = S.vbid * WPB + threadIdx.x / 64;
+    const uint32_t GW = S.npart * WPB;
+    const __amdgpu_buffer_rsrc_t core = make_rsrc(a.core, a.nvec * 16u);
+    uint32_t sn = 0, vn = 0;
+    for (uint32_t t = gw * 64 * U; t < a.nvec; t += GW * 64 * U) {
+        uint4 d[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t i = t + k * 64 + lane;
+            d[k] = i < a.nvec ? load_quad_coherent(core, i) : make_uint4(kSent, kSent, kSent, kSent);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t v0 = 4 * (t + k * 64 + lane);
+            select_quad(a, S, L, d[k], v0, mn, sn);
+            if (compact) {
+                // padding entries (index >= V) hold the sentinel and are never survivors
+                surv_push(a, S, vn, d[k].x > L && v0 < a.V, v0, dst);
+                surv_push(a, S, vn, d[k].y > L && v0 + 1 < a.V, v0 + 1, dst);
+                surv_push(a, S, vn, d[k].z > L && v0 + 2 < a.V, v0 + 2, dst);
+                surv_push(a, S, vn, d[k].w > L && v0 + 3 < a.V, v0 + 3, dst);
+            }
+        }
+    }
+    flush_stage(a, S, sn, false, false);
+    if (compact) flush_surv(a, S, vn, dst);
+}
+
+// Scan of an alive list: ids are read coalesced, their degrees gathered.
+template <int BLOCK>
+__device__ void scan_list(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, const uint32_t* list,
+                          uint32_t len, bool compact, uint32_t dst) {
+    constexpr uint32_t WPB = BLOCK / 64;
+    constexpr uint32_t U = 4;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = S.vbid * WPB + threadIdx.x / 64;
+    const uint32_t GW = S.npart * WPB;
+    uint32_t sn = 0, vn = 0;
+    for (uint32_t t = gw * 64 * U; t < len; t += GW * 64 * U) {
+        uint32_t id[U], d[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t i = t + k * 64 + lane;
+            id[k] = i < len ? ld_rlx(list + i) : kSent;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) d[k] = id[k] != kSent ? ld_rlx(a.core + id[k]) : kSent;
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const bool valid = id[k] != kSent;
+            if (valid && d[k] > L && d[k] < mn) mn = d[k];
+            stage_push(a, S, sn, valid && d[k] == L, id[k], false, false);
+            if (compact) surv_push(a, S, vn, valid && d[k] > L, id[k], dst);
+        }
+    }
+    flush_stage(a, S, sn, false, false);
+    if (compact) flush_surv(a, S, vn, dst);
+}
+
+__device__ __forceinline__ bool level_complete(const PeelArgs& a) {
+    const uint32_t vd = ld_rlx(cw(a, A_VDONE));
+    const uint32_t cd = ld_rlx(cw(a, A_CDONE));
+    order_point();  // the done counters are read before the tails
+    const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
+    const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
+    return vd == vt && cd == ct;
+}
+
+__device__ __forceinline__ void signal_level_done(const PeelArgs& a, uint32_t gen) {
+    for (uint32_t x = 0; x < kXcds; ++x) st_rlx(cw(a, A_LDONE + x), gen);
+}
+
+__device__ __forceinline__ void idle_sleep(uint32_t nap, uint32_t poll_max) {
+    if (nap < 8 || poll_max == 0) {
+        __builtin_amdgcn_s_sleep(2);
+    } else if (nap < 32 || poll_max == 1) {
+        __builtin_amdgcn_s_sleep(8);
+    } else if (nap < 128 || poll_max == 2) {
+        __builtin_amdgcn_s_sleep(24);
+    } else {
+        __builtin_amdgcn_s_sleep(64);
+    }
+}
+
+// The processing phase of level L for this workgroup. First its static share of the scan's
+// frontier (chunks and 64-vertex batches dealt round-robin over every wave of the grid, as in
+// peel_persistent's first sub-round); then claim, process, append, count, until the level's done
+// flag carries generation `gen`.
+template <int BLOCK, bool PREFETCH>
+__device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen, uint32_t& mn) {
+    constexpr uint32_t WPB = BLOCK / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wid = threadIdx.x / 64;
+    uint32_t* ldone = cw(a, A_LDONE + S.vbid % kXcds);
+    uint32_t sn = 0;
+    {
+        const uint32_t gw = S.vbid * WPB + wid;
+        const uint32_t GW = S.npart * WPB;
+        const uint32_t vbeg = ld_rlx(cw(a, A_SVBEG)), vend = ld_rlx(cw(a, A_SVEND));
+        const uint32_t cbeg = ld_rlx(cw(a, A_SCBEG)), cend = ld_rlx(cw(a, A_SCEND));
+        if (threadIdx.x == 0) {
+            S.vcount = 0;
+            S.ccount = 0;
+        }
+        __syncthreads();
+        uint32_t nc = 0, nv = 0;
+        for (uint32_t c = cbeg + gw; c < cend; c += GW) {
+            uint64_t ch = 0;
+            if (lane == 0) ch = ld_rlx64(a.chunks + c);
+            ch = (uint64_t)bcast_first((uint32_t)ch) | ((uint64_t)bcast_first((uint32_t)(ch >> 32)) << 32);
+            process_chunk<BLOCK, PREFETCH>(a, S, L, ch, mn, sn, false);
+            ++nc;
+        }
+        for (uint32_t b = vbeg + 64 * gw; b < vend; b += 64 * GW) {
+            const bool mine = b + lane < vend;
+            const uint32_t v = mine ? ld_rlx(a.order + b + lane) : 0;
+            process_vertices<BLOCK, PREFETCH>(a, S, L, mine, v, mn, sn, false);
+            nv += min(64u, vend - b);
+        }
+        flush_stage(a, S, sn, true, false);
+        drain_memory();
+        if (lane == 0 && (nc | nv)) {
+            __hip_atomic_fetch_add(&S.vcount, nv, __ATOMIC_RELAXED, KC_WG);
+            __hip_atomic_fetch_add(&S.ccount, nc, __ATOMIC_RELAXED, KC_WG);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && (S.vcount | S.ccount)) {
+            if (S.vcount) add_rlx(cw(a, A_VDONE), S.vcount);
+            if (S.ccount) add_rlx(cw(a, A_CDONE), S.ccount);
+            order_point();
+            if (level_complete(a)) signal_level_done(a, gen);
+        }
+        __syncthreads();
+    }
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t kind = 3, beg = 0, n = 0, nap = 0;
+            Waiter w;
+            while (true) {
+                const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
+                const uint32_t ch = ld_rlx(cw(a, A_CHEAD));
+                if (ch < ct) {
+                    const uint32_t k = min(WPB, ct - ch);
+                    if (cas_rlx(cw(a, A_CHEAD), ch, ch + k)) {
+                        kind = 1, beg = ch, n = k;
+                        break;
+                    }
+                    continue;
+                }
+                const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
+                const uint32_t vh = ld_rlx(cw(a, A_VHEAD));
+                if (vh < vt) {
+                    const uint32_t k = min((uint32_t)BLOCK, vt - vh);
+                    if (cas_rlx(cw(a, A_VHEAD), vh, vh + k)) {
+                        kind = 2, beg = vh, n = k;
+                        break;
+                    }
+                    continue;
+                }
+                if (ld_rlx(ldone) == gen) break;
+                if ((++nap & 7) == 0 && level_complete(a)) {
+                    signal_level_done(a, gen);
+                    break;
+                }
+                if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                idle_sleep(nap, a.poll_max);
+            }
+            S.kind = kind;
+            S.beg = beg;
+            S.n = n;
+        }
+        __syncthreads();
+        const uint32_t kind = S.kind, beg = S.beg, n = S.n;
+        if (kind == 3) break;
+        if (kind == 1) {
+            if (wid < n) {
+                uint64_t ch = 0;
+                if (lane == 0) {
+                    Waiter w;
+                    while ((ch = ld_rlx64(a.chunks + beg + wid)) == 0) {
+                        if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                ch = (uint64_t)bcast_first((uint32_t)ch) | ((uint64_t)bcast_first((uint32_t)(ch >> 32)) << 32);
+                if (ch) process_chunk<BLOCK, PREFETCH>(a, S, L, ch, mn, sn, false);
+            }
+        } else {
+            for (uint32_t j0 = wid * 64; j0 < n; j0 += BLOCK) {
+                const bool mine = j0 + lane < n;
+                uint32_t v = kSent;
+                if (mine) {
+                    Waiter w;
+                    while ((v = ld_rlx(a.order + beg + j0 + lane)) == kSent) {
+                        if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                process_vertices<BLOCK, PREFETCH>(a, S, L, mine && v != kSent, v, mn, sn, false);
+            }
+        }
+        // crossings go to order[] (and the chunk list) before this work counts as done
+        flush_stage(a, S, sn, true, false);
+        drain_memory();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            add_rlx(cw(a, kind == 1 ? A_CDONE : A_VDONE), n);
+            order_point();
+            if (level_complete(a)) signal_level_done(a, gen);
+        }
+        // S.kind is rewritten only by thread 0 after this barrier's readers are done
+        __syncthreads();
+    }
+}
+
+// after the scan: account the scan, adopt a freshly built alive list; an empty level is done
+template <int BLOCK>
+__device__ void hook_scan_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen) {
+    if (threadIdx.x != 0) return;
+    const uint32_t mode = ld_rlx(cw(a, A_MODE));
+    if (mode == 0) {
+        add_rlx(cw(a, A_SCAN_DENSE), a.nvec * 4);
+    } else {
+        add_rlx(cw(a, A_SCAN_LIST), ld_rlx(cw(a, A_LEN)));
+    }
+    if (ld_rlx(cw(a, A_COMPACT))) {
+        const uint32_t dst = ld_rlx(cw(a, A_DST));
+        const uint32_t len = ld_rlx(cw(a, A_ALEN0 + dst));
+        add_rlx(cw(a, A_SURV), len);
+        st_rlx(cw(a, A_MODE), 1u);
+        st_rlx(cw(a, A_LIST), dst);
+        st_rlx(cw(a, A_LEN), len);
+        st_rlx(cw(a, A_COMPACT), 0u);
+    }
+    // the scan's frontier is dealt statically; the queues open behind it
+    const uint32_t vh = ld_rlx(cw(a, A_VHEAD)), vt = ld_rlx(cw(a, C_OTAIL));
+    const uint32_t ch = ld_rlx(cw(a, A_CHEAD)), ct = ld_rlx(cw(a, C_CTAIL));
+    st_rlx(cw(a, A_SVBEG), vh);
+    st_rlx(cw(a, A_SVEND), vt);
+    st_rlx(cw(a, A_SCBEG), ch);
+    st_rlx(cw(a, A_SCEND), ct);
+    st_rlx(cw(a, A_VHEAD), vt);
+    st_rlx(cw(a, A_CHEAD), ct);
+    if (level_complete(a)) signal_level_done(a, gen);
+    trace_rec(a, TR_SCAN, 0, L);
+}
+
+// level end: next level, shell offsets, compaction of the next scan
+template <int BLOCK>
+__device__ void hook_level_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L) {
+    uint32_t m = kSent;
+    for (uint32_t i = threadIdx.x; i < S.npart; i += BLOCK) m = min(m, ld_rlx(a.wgmin + i));
+    m = wave_min(m);
+    if (lane_id() == 0) S.red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 0; w < BLOCK / 64; ++w) m = min(m, S.red[w]);
+    const uint32_t t = ld_rlx(cw(a, C_OTAIL));
+    uint32_t phase = PH_NEXT_LEVEL, next = L + 1;
+    if (!(a.flags & FLAG_NO_LEVEL_SKIP) && m != kSent && m > next) next = m;
+    add_rlx(cw(a, C_LEVELS), 1u);
+    if (t >= a.V) {
+        a.shell[L + 1] = t;
+        phase = PH_DONE;
+        next = L;
+    } else if (next > a.dmax) {
+        __hip_atomic_fetch_max(cw(a, C_ERR), 2u, __ATOMIC_RELAXED, KC_AGENT);
+        phase = PH_DONE;
+    } else {
+        for (uint32_t k = L + 1; k <= next; ++k) a.shell[k] = t;
+        // the next scan compacts when few of the entries it would read are still unpeeled
+        const uint32_t mode = ld_rlx(cw(a, A_MODE));
+        const uint64_t reads = mode ? ld_rlx(cw(a, A_LEN)) : (uint64_t)a.nvec * 4;
+        const uint64_t unpeeled = a.V - t;
+        if (a.compact_ratio && unpeeled * a.compact_ratio < reads) {
+            const uint32_t dst = mode ? ld_rlx(cw(a, A_LIST)) ^ 1u : 0u;
+            st_rlx(cw(a, A_ALEN0 + dst), 0u);
+            st_rlx(cw(a, A_DST), dst);
+            st_rlx(cw(a, A_COMPACT), 1u);
+        }
+    }
+    st_rlx(cw(a, A_PHASE), phase);
+    st_rlx(cw(a, A_NEXT), next);
+    trace_rec(a, TR_LEVEL, 0, L);
+}
+
+template <int BLOCK, bool PREFETCH>
+__global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
+    __shared__ WgStateA<BLOCK> S;
+    if (!register_workgroup<BLOCK>(a, S)) return;
+    uint32_t L = 0, gen = 0;
+    uint32_t mn = kSent;
+    if (threadIdx.x == 0) S.mn = kSent;
+    __syncthreads();
+    while (true) {
+        // control words written by the hooks are read here, before this workgroup arrives at the
+        // next barrier (the only place they change)
+        const uint32_t mode = ld_rlx(cw(a, A_MODE));
+        const uint32_t list = ld_rlx(cw(a, A_LIST));
+        const uint32_t len = ld_rlx(cw(a, A_LEN));
+        const bool compact = ld_rlx(cw(a, A_COMPACT)) != 0;
+        const uint32_t dst = ld_rlx(cw(a, A_DST));
+        if (threadIdx.x == 0) {
+            st_rlx(a.dbg + S.vbid * 8 + 1, L);
+            st_rlx(a.dbg + S.vbid * 8 + 2, 1u);
+        }
+        if (mode == 0) {
+            scan_dense<BLOCK>(a, S, L, mn, compact, dst);
+        } else {
+            scan_list<BLOCK>(a, S, L, mn, a.alive[list], len, compact, dst);
+        }
+        ++gen;
+        grid_barrier<BLOCK>(a, S, [&] { hook_scan_async<BLOCK>(a, S, L, gen); });
+        if (ld_rlx(cw(a, C_ERR))) break;
+        if (threadIdx.x == 0) st_rlx(a.dbg + S.vbid * 8 + 2, 2u);
+        process_async<BLOCK, PREFETCH>(a, S, L, gen, mn);
+        // publish this workgroup's smallest degree above L seen in the level
+        mn = wave_min(mn);
+        if (lane_id() == 0) __hip_atomic_fetch_min(&S.mn, mn, __ATOMIC_RELAXED, KC_WG);
+        __syncthreads();
+        if (threadIdx.x == 0) st_rlx(a.wgmin + S.vbid, S.mn);
+        grid_barrier<BLOCK>(a, S, [&] { hook_level_async<BLOCK>(a, S, L); });
+        if (ld_rlx(cw(a, C_ERR))) break;
+        if (ld_rlx(cw(a, A_PHASE)) == PH_DONE) break;
+        L = ld_rlx(cw(a, A_NEXT));
+        mn = kSent;
+        __syncthreads();
+        if (threadIdx.x == 0) S.mn = kSent;
+        __syncthreads();
+    }
+    if (S.vbid == 0 && threadIdx.x == 0) {
+        uint32_t* r = cw(a, C_RESULT);
+        r[R_ERR] = ld_rlx(cw(a, C_ERR));
+        r[R_KMAX] = L;
+        r[R_LEVELS] = ld_rlx(cw(a, C_LEVELS));
+        r[R_SUBROUNDS] = 0;
+        r[R_NPART] = S.npart;
+        r[R_NTRACE] = ld_rlx(cw(a, C_NTRACE));
+        r[R_WORDS] = ld_rlx(cw(a, A_SCAN_DENSE));
+        r[R_WORDS + 1] = ld_rlx(cw(a, A_SCAN_LIST));
+        r[R_WORDS + 2] = ld_rlx(cw(a, A_SURV));
+        r[R_WORDS + 3] = ld_rlx(cw(a, C_CTAIL));
+    }
+}
+
+}  // namespace kcdev

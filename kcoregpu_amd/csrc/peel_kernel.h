@@ -99,6 +99,11 @@ struct PeelArgs {
     uint32_t trace_cap;
     uint64_t time_limit;   // wall-clock ticks one wait may take
     uint64_t reg_window;   // wall-clock ticks without a new check-in that close registration
+    // cascade-queue kernel (peel_async) only
+    uint32_t* alive[2];    // padded V each: lists of vertices still unpeeled, for compacted scans
+    uint32_t async_mode;   // 1: crossings are consumed from order[] itself (no work[] spill)
+    uint32_t compact_ratio;  // compact when unpeeled * ratio < entries the scan reads
+    uint32_t poll_max;     // longest sleep between polls of an idle workgroup (0..3)
 };
 
 constexpr uint32_t FLAG_NO_LEVEL_SKIP = 1u;
@@ -272,7 +277,7 @@ __device__ void flush_stage(const PeelArgs& a, WgState<BLOCK>& S, uint32_t& n, b
         const uint32_t d = has ? a.deg0[u] : 0;  // original degree = adjacency length
         const bool heavy = has && d > a.heavy;
         if (__ballot(heavy)) push_chunks(a, heavy, heavy ? a.off[u] : 0, d);
-        if (!crossings) continue;
+        if (!crossings || a.async_mode) continue;
         const bool light = has && !heavy;
         bool spill = light;
         if (local && __ballot(light)) {
@@ -694,6 +699,7 @@ __global__ __launch_bounds__(BLOCK) void peel_persistent(PeelArgs a) {
         r[R_SUBROUNDS] = ld_rlx(cw(a, C_SUBROUNDS));
         r[R_NPART] = S.npart;
         r[R_NTRACE] = ld_rlx(cw(a, C_NTRACE));
+        r[R_WORDS + 3] = ld_rlx(cw(a, C_CTAIL));
     }
 }
 
