@@ -196,6 +196,9 @@ kc_status kc_tune_batch(kc_ctx* ctx, kc_batch* b, const kc_tune_cfg* cfg, kc_par
  * leave the running kernel individually. When core_out is pinned host memory (kc_host_alloc)
  * the kernel writes the core numbers there itself; otherwise kc_serve_poll copies them down.
  * kc_serve_poll reports a request once its core numbers are in core_out and frees its region.
+ * Requests of at least large_arcs arcs are peeled instead by the single-graph kernel over many
+ * workgroups, on their own streams next to the resident kernel (charged to the same budget as
+ * max(8 m + 16 n, the single-graph state: about 4.7 arcs + 28 n bytes + 2 MiB)).
  * The CSR arrays and core_out must stay valid until the request's completion is returned.
  * Not thread-safe: one host thread drives a server.
  * The kernel never outlives the process: idle workgroups exit when the library's heartbeat
@@ -208,6 +211,11 @@ typedef struct {
     uint32_t workgroups;       /* resident serving workgroups; 0 = 2 per CU (256) or 1 per CU */
     uint32_t flags;            /* KC_FLAG_NO_LEVEL_SKIP */
     uint32_t lease_ms;         /* heartbeat lease of the resident kernel; 0 = 5000 */
+    uint32_t large_arcs;       /* requests with at least this many arcs go to a large-request lane
+                                  (the single-graph kernel over many workgroups, next to the
+                                  resident kernel) instead of one workgroup; 0 = 1M, UINT32_MAX =
+                                  never */
+    uint32_t large_lanes;      /* large requests peeled at once; 0 = 2 */
 } kc_serve_cfg;
 typedef struct {
     uint64_t req_id;

@@ -79,10 +79,10 @@ int kcg_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c,
 
 /* Served-workload requests: request i of stream `seed` asks for m edges, m lognormal with the
  * given median and sigma (Box-Muller on counter-based draws), truncated to [mmin, mmax] by
- * redrawing, and a seeded coin flip picks the family (kind 0: R-MAT (0.57, 0.19, 0.19) on the
- * smallest 2^scale >= m/8 vertices with m draws; kind 1: Chung-Lu, exponent 2.1, n = m/6,
- * average degree 2m/n, weights clipped to [1, min(2800, n/2)]). Deduplication and self-loop
- * removal leave somewhat fewer than m distinct edges. */
+ * redrawing, and a seeded coin flip picks the family (kind 0: R-MAT (0.57, 0.19, 0.19) on
+ * 2^scale vertices, scale = ceil(log2(m/16)), with m draws; kind 1: Chung-Lu, exponent 2.1,
+ * n = ceil(2m/12), average degree 2m/n, weights clipped to [1, min(2800, n/2)]). Deduplication
+ * and self-loop removal leave somewhat fewer than m distinct edges. */
 int kcg_request_shape(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
                       uint64_t* m_out, int* kind_out);
 int kcg_gen_request(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,

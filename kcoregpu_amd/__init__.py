@@ -144,7 +144,7 @@ class Graph:
         return cls._make(N.host().kcg_gen_rmat, scale, nd, a, b, c, seed, nthreads)
 
     @classmethod
-    def request(cls, seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.5,
+    def request(cls, seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.0,
                 mmin: int = 1000, mmax: int = 50_000_000, nthreads: int = 0) -> "Graph":
         """Request i of a served-workload stream (R-MAT or Chung-Lu by a seeded coin flip, edge
         count lognormal around median_m); .kind is 0 for R-MAT, 1 for Chung-Lu."""
@@ -155,7 +155,7 @@ class Graph:
         return g
 
     @staticmethod
-    def request_shape(seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.5,
+    def request_shape(seed: int, i: int, median_m: float = 50000.0, sigma: float = 1.0,
                       mmin: int = 1000, mmax: int = 50_000_000):
         m, kind = C.c_uint64(), C.c_int()
         st = N.host().kcg_request_shape(seed, i, median_m, sigma, mmin, mmax, C.byref(m), C.byref(kind))
@@ -328,7 +328,8 @@ def _tune(fn, eng, handle, reps, flags, cache_path, single):
     out = {"block": best.block, "flags": best.flags, "ms": ms.value, "trials": n.value}
     if single:
         out.update(variant=best.variant, heavy_threshold=best.heavy_threshold,
-                   chunk_arcs=best.chunk_arcs, lds_depth=best.lds_depth)
+                   chunk_arcs=best.chunk_arcs, lds_depth=best.lds_depth,
+                   compact_ratio=best.compact_ratio, poll_max=best.poll_max)
     return out
 
 
@@ -361,11 +362,13 @@ class Server:
     in pinned buffers written by the kernel itself; otherwise they are copied down."""
 
     def __init__(self, eng: "Engine", hbm_budget: int = 0, ring_slots: int = 0, block: int = 0,
-                 workgroups: int = 0, flags: int = 0, lease_ms: int = 0, pinned: bool = True):
+                 workgroups: int = 0, flags: int = 0, lease_ms: int = 0, pinned: bool = True,
+                 large_arcs: int = 0, large_lanes: int = 0):
         self.eng = eng
         self._s = C.c_void_p()
         self.pinned = pinned
-        cfg = N.kc_serve_cfg(hbm_budget, ring_slots, block, workgroups, flags, lease_ms)
+        cfg = N.kc_serve_cfg(hbm_budget, ring_slots, block, workgroups, flags, lease_ms, large_arcs,
+                             large_lanes)
         _check(N.hip().kc_serve_open(eng._c, C.byref(cfg), C.byref(self._s)), "kc_serve_open")
         self._live = {}  # req_id -> (graph, core buffer): kept alive until completion
         self._next = 0

@@ -65,7 +65,8 @@ class kc_stats(C.Structure):
 
 class kc_serve_cfg(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("ring_slots", C.c_uint32), ("block", C.c_uint32),
-                ("workgroups", C.c_uint32), ("flags", C.c_uint32), ("lease_ms", C.c_uint32)]
+                ("workgroups", C.c_uint32), ("flags", C.c_uint32), ("lease_ms", C.c_uint32),
+                ("large_arcs", C.c_uint32), ("large_lanes", C.c_uint32)]
 
 
 class kc_completion(C.Structure):

@@ -249,9 +249,10 @@ int kcg_gen_request(uint64_t seed, uint64_t i, double median_m, double sigma, ui
     const uint64_t gseed = kcg::splitmix64(seed * 0x9E3779B97F4A7C15ull + i) ^ 0x5EEDull;
     auto* g = new kcg_graph;
     if (kind == 0) {
-        // R-MAT, edge factor 8: 2^scale >= m/8 vertices, m draws (a, b, c) = (0.57, 0.19, 0.19)
+        // R-MAT on 2^scale vertices, scale = ceil(log2(m / 16)) (edge factor 16), m draws,
+        // (a, b, c) = (0.57, 0.19, 0.19)
         uint32_t scale = 6;
-        while (scale < 30 && ((uint64_t)8 << scale) < m) ++scale;
+        while (scale < 30 && ((uint64_t)16 << scale) < m) ++scale;
         rc = kcg::gen_rmat(scale, m, 0.57, 0.19, 0.19, gseed, nthreads, g->csr);
     } else {
         // Chung-Lu, exponent 2.1, average degree 12 -> n = m/6, weights clipped to [1, min(2800, n/2)]

@@ -11,7 +11,7 @@
 //   --core-out PATH   write the core numbers as raw little-endian uint32[V]
 //   --json            write ./output/pkc-kcore-<edgelist> like Graph::writeKCoreToDisk
 //   --order-out PATH  write the peel order (uint32[V]) and PATH.shells (k-shell starts)
-//   --variant NAME    run only one variant: wave | lds | prefetch | stepwise
+//   --variant NAME    run only one variant: wave | lds | prefetch | async | auto | stepwise
 //   --reps N          repeat each variant N times (the reference uses 1)
 //   --stats           print a JSON stats line per run
 //   --output-dir DIR  directory for the serialized cache and JSON (default ./output/)
@@ -119,6 +119,8 @@ int main(int argc, char* argv[]) {
         int v = only == "wave" ? KC_VARIANT_WAVE
               : only == "lds" ? KC_VARIANT_LDS
               : only == "prefetch" ? KC_VARIANT_PREFETCH
+              : only == "async" ? KC_VARIANT_ASYNC
+              : only == "auto" ? KC_VARIANT_AUTO
               : only == "stepwise" ? KC_VARIANT_STEPWISE : -1;
         if (v < 0) {
             std::cerr << "unknown variant " << only << std::endl;

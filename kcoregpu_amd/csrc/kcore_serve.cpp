@@ -8,6 +8,7 @@
 //   kcore_serve [--mode served|plain] [--rate R] [--seconds T] [--pool P] [--seed S]
 //               [--median M] [--sigma S] [--mmin A] [--mmax B] [--budget-gb G] [--block B]
 //               [--workgroups W] [--slots N] [--expect FILE] [--drain-limit S] [--closed N]
+//               [--max-backlog N] [--large-arcs A] [--large-lanes K]
 //
 // --closed N replaces the Poisson stream by N requests that all arrive at time 0: the run then
 // measures the path's capacity (graphs/s), and its latencies are queueing times.
@@ -30,7 +31,7 @@ namespace {
 
 struct Opts {
     std::string mode = "served";
-    double rate = 2000, seconds = 10, median = 50000, sigma = 1.5;
+    double rate = 2000, seconds = 10, median = 50000, sigma = 1.0;
     uint64_t mmin = 1000, mmax = 50000000;
     uint32_t pool = 256;
     uint64_t seed = 1;
@@ -40,6 +41,7 @@ struct Opts {
     double drain_limit = 120;
     uint64_t closed = 0;
     uint64_t max_backlog = 20000;
+    uint32_t large_arcs = 0, large_lanes = 0;
 };
 
 struct PoolGraph {
@@ -112,6 +114,8 @@ int main(int argc, char** argv) {
         else if (a == "--drain-limit") o.drain_limit = std::atof(val());
         else if (a == "--closed") o.closed = std::strtoull(val(), nullptr, 10);
         else if (a == "--max-backlog") o.max_backlog = std::strtoull(val(), nullptr, 10);
+        else if (a == "--large-arcs") o.large_arcs = (uint32_t)std::strtoul(val(), nullptr, 10);
+        else if (a == "--large-lanes") o.large_lanes = (uint32_t)std::atoi(val());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -232,6 +236,8 @@ int main(int argc, char** argv) {
         cfg.block = o.block;
         cfg.workgroups = o.workgroups;
         cfg.ring_slots = o.slots;
+        cfg.large_arcs = o.large_arcs;
+        cfg.large_lanes = o.large_lanes;
         st = kc_serve_open(ctx, &cfg, &g_srv);
         if (st) die("kc_serve_open", st);
         std::vector<uint32_t*> outbuf(N, nullptr);

@@ -12,6 +12,11 @@
 // pinned host memory gets its core numbers written there by the kernel itself; otherwise
 // kc_serve_poll copies them down.
 //
+// Large requests (at least large_arcs arcs) would hold one workgroup for long: they go to a
+// large-request lane instead, where a worker thread with its own context and stream uploads the
+// request, peels it with the single-graph kernel over many workgroups (running beside the resident
+// kernel) and downloads the core numbers; large_lanes workers run at once.
+//
 // Lifetime safety: the kernel never outlives its process. A heartbeat thread advances a word in
 // the control page while the server is open; idle kernel workgroups exit when it stops advancing
 // for lease_ms, and an exit handler stops every open server before the HIP runtime shuts down.
@@ -55,6 +60,7 @@ struct Req {
     double t_submit = 0, t_admit = 0;
     hipEvent_t ev = nullptr;
     int state = 0;  // 0 free, 1 on the device, 2 finishing
+    bool large = false;
 };
 
 // first-fit allocator over [0, size)
@@ -141,7 +147,23 @@ struct kc_server {
     std::mutex hb_m;
     std::condition_variable hb_cv;
     bool hb_quit = false;
+    // large-request lane
+    std::deque<Req> lqueue;         // waiting for a lane (arrival order), owned by the host thread
+    std::vector<std::thread> lanes;
+    std::mutex lm;
+    std::condition_variable lcv;
+    std::deque<Req> ljobs;          // admitted, waiting for a worker
+    struct LaneDone {
+        kc_completion c;
+        uint64_t base, charge;      // the arena charge to release
+    };
+    std::vector<LaneDone> ldone;
+    uint32_t lbusy = 0;             // admitted large requests not yet reported (host thread)
+    bool lquit = false;
+    kc_status lerr = KC_OK;         // first worker failure (context could not be opened)
 };
+
+static void lanes_stop(kc_server* s);
 
 namespace {
 
@@ -168,6 +190,7 @@ void serve_atexit() {
         v = g_servers;
     }
     for (kc_server* s : v) {
+        lanes_stop(s);
         if (!s->running) continue;
         __atomic_store_n(s->h_ctl + kHStop, 1u, __ATOMIC_RELEASE);
         hb_stop(s);
@@ -192,11 +215,73 @@ void reg_del(kc_server* s) {
 
 }  // namespace
 
+// A large-request lane: its own context (stream, events), one request at a time through the
+// single-graph path; results are reported through s->ldone.
+static void lane_main(kc_server* s) {
+    kc_ctx* c = nullptr;
+    const kc_status os = kc_open(s->ctx->device, &c);
+    std::unique_lock<std::mutex> l(s->lm);
+    if (os != KC_OK && s->lerr == KC_OK) s->lerr = os;
+    while (true) {
+        s->lcv.wait(l, [&] { return s->lquit || !s->ljobs.empty(); });
+        if (s->ljobs.empty()) break;  // quit, and nothing left to run
+        Req r = s->ljobs.front();
+        s->ljobs.pop_front();
+        l.unlock();
+        kc_completion cp{};
+        cp.req_id = r.id;
+        cp.t_submit = r.t_submit;
+        cp.t_admit = r.t_admit;
+        kc_status st = c ? KC_OK : os;
+        if (c) {
+            kc_csr h{r.V, r.arcs, r.off, r.nbr, nullptr};
+            kc_graph* g = nullptr;
+            st = kc_graph_upload(c, &h, &g);
+            if (st == KC_OK) {
+                kc_params p{};
+                p.flags = s->cfg.flags;
+                kc_stats stt{};
+                uint32_t kmax = 0;
+                st = kc_peel(c, g, &p, r.core_out, &kmax, &stt);
+                cp.kmax = kmax;
+                cp.levels = stt.levels;
+                cp.rounds = stt.subrounds;
+                kc_graph_free(g);
+            }
+        }
+        cp.status = st;
+        cp.t_done = now_s();
+        l.lock();
+        s->ldone.push_back({cp, r.base, r.charge});
+    }
+    l.unlock();
+    if (c) kc_close(c);
+}
+
+static void lanes_stop(kc_server* s) {
+    {
+        std::lock_guard<std::mutex> l(s->lm);
+        s->lquit = true;
+    }
+    s->lcv.notify_all();
+    for (auto& t : s->lanes)
+        if (t.joinable()) t.join();
+    s->lanes.clear();
+}
+
+// charge of a large request: the single-graph state (offsets, neighbours, five vertex arrays, chunk
+// slots of 5/8 byte per arc, two alive lists, control and trace blocks)
+static uint64_t large_charge(uint32_t V, uint32_t arcs) {
+    const uint64_t state = 4ull * (V + 1) + 4ull * arcs + 28ull * V + (5ull * arcs) / 8 + (2ull << 20);
+    return std::max<uint64_t>(state, 4ull * arcs + 16ull * V);
+}
+
 // Stops the serving kernel (the stop flag travels on the copy stream, after every publication,
 // so every admitted request still runs) and frees everything. Returns KC_ERR_DEVICE when the
 // kernel did not exit within the wait limit; its memory is then left allocated (the immediate
 // stop flag is raised and the heartbeat stopped, so the kernel still ends).
 static kc_status serve_release(kc_server* s) {
+    lanes_stop(s);  // every admitted large request finishes first
     if (s->running) {
         *s->h_one = 1;
         hipError_t e = hipMemcpyAsync(s->d_head + kStopWord, s->h_one, 4, hipMemcpyHostToDevice, s->cstream);
@@ -229,6 +314,23 @@ static kc_status serve_release(kc_server* s) {
 
 // admit queued requests in arrival order while their slot is free and the arena has room
 static kc_status serve_admit(kc_server* s) {
+    // large requests: a free lane and room in the arena
+    bool handed = false;
+    while (!s->lqueue.empty() && s->lbusy < s->cfg.large_lanes) {
+        Req& q = s->lqueue.front();
+        uint64_t base;
+        if (!s->alloc.alloc(q.charge, &base)) break;
+        q.base = base;
+        q.t_admit = now_s();
+        {
+            std::lock_guard<std::mutex> l(s->lm);
+            s->ljobs.push_back(q);
+        }
+        s->lqueue.pop_front();
+        s->lbusy++;
+        handed = true;
+    }
+    if (handed) s->lcv.notify_all();
     const uint32_t R = s->cfg.ring_slots;
     s->fresh.clear();
     while (!s->queue.empty()) {
@@ -290,6 +392,8 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     if (!s->cfg.ring_slots) s->cfg.ring_slots = 1024;
     if (!s->cfg.block) s->cfg.block = 256;
     if (!s->cfg.lease_ms) s->cfg.lease_ms = 5000;
+    if (!s->cfg.large_arcs) s->cfg.large_arcs = 1u << 20;
+    if (!s->cfg.large_lanes) s->cfg.large_lanes = 2;
     if (s->cfg.block != 256 && s->cfg.block != 512 && s->cfg.block != 1024) {
         delete s;
         return KC_ERR_ARG;
@@ -352,6 +456,8 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     }
     if (e) return fail(e);
     s->running = true;
+    if (s->cfg.large_arcs != 0xFFFFFFFFu)
+        for (uint32_t i = 0; i < s->cfg.large_lanes; ++i) s->lanes.emplace_back(lane_main, s);
     *out = s;
     return KC_OK;
 }
@@ -369,11 +475,13 @@ kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32
     q.nbr = g->neighbors;
     q.core_out = core_out;
     q.out_pinned = core_out && is_pinned(core_out);
+    q.large = !s->lanes.empty() && q.arcs >= s->cfg.large_arcs;
     q.r = serve_region(q.V, q.arcs);
-    q.charge = std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
+    q.charge = q.large ? large_charge(q.V, q.arcs)
+                       : std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
     if (q.charge > s->cfg.hbm_budget_bytes) return KC_ERR_CAPACITY;
     q.t_submit = now_s();
-    s->queue.push_back(q);
+    (q.large ? s->lqueue : s->queue).push_back(q);
     return serve_admit(s);
 }
 
@@ -382,6 +490,16 @@ kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t
     *n_out = 0;
     if (!s->running) return KC_ERR_DEVICE;
     uint32_t n = 0;
+    if (s->lbusy) {
+        std::lock_guard<std::mutex> l(s->lm);
+        while (!s->ldone.empty() && n < cap) {
+            const kc_server::LaneDone d = s->ldone.back();
+            s->ldone.pop_back();
+            out[n++] = d.c;
+            s->alloc.release(d.base, d.charge);
+            s->lbusy--;
+        }
+    }
     for (uint32_t sl = 0; sl < s->cfg.ring_slots && s->inflight; ++sl) {
         Req& r = s->slot[sl];
         if (r.state == 1) {
@@ -426,8 +544,8 @@ kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t
 
 kc_status kc_serve_pending(const kc_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* arena_used) {
     if (!s) return KC_ERR_ARG;
-    if (queued) *queued = (uint32_t)s->queue.size();
-    if (inflight) *inflight = s->inflight;
+    if (queued) *queued = (uint32_t)(s->queue.size() + s->lqueue.size());
+    if (inflight) *inflight = s->inflight + s->lbusy;
     if (arena_used) *arena_used = s->alloc.used;
     return KC_OK;
 }

@@ -1,13 +1,14 @@
 // tune.cpp — the autotuner: picks kc_params for a resident graph or batch by timing real peels.
 //
-// Single graph: coordinate descent from the defaults over workgroup size, variant, the degree
+// Single graph: coordinate descent from the defaults over variant, workgroup size, the degree
 // above which a vertex's arcs are split into chunks shared by many waves (the wave/block
-// threshold), chunk length and the LDS frontier depth; each axis keeps its fastest value (median
-// of `reps` timed peels, device time) before the next axis is tried. Batch: workgroup size.
+// threshold), chunk length, then the LDS frontier depth (sub-round variants) or the alive-list
+// compaction ratio and idle poll step (cascade-queue variant); each axis keeps its fastest value
+// (median of `reps` timed peels, device time) before the next axis is tried. Batch: workgroup size.
 // Every candidate produces identical core numbers, so tuning changes speed only.
 //
 // Results can be kept in a text cache, one line per key:
-//   peel <V> <arcs> <dmax> <flags> <arch> <cus> : <variant> <block> <heavy> <chunk> <lds> <ms>
+//   peel <V> <arcs> <dmax> <flags> <arch> <cus> : <variant> <block> <heavy> <chunk> <lds> <compact> <poll> <ms>
 //   batch <G> <vertices> <arcs> <flags> <arch> <cus> : <block> <ms>
 // A cached key is answered without timing anything.
 #include <hip/hip_runtime.h>
@@ -79,8 +80,8 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
     if (!hit.empty()) {
         kc_params p{};
         double ms = 0;
-        if (std::sscanf(hit.c_str(), "%d %u %u %u %u %lf", &p.variant, &p.block, &p.heavy_threshold,
-                        &p.chunk_arcs, &p.lds_depth, &ms) == 6) {
+        if (std::sscanf(hit.c_str(), "%d %u %u %u %u %u %u %lf", &p.variant, &p.block, &p.heavy_threshold,
+                        &p.chunk_arcs, &p.lds_depth, &p.compact_ratio, &p.poll_max, &ms) == 8) {
             p.flags = tc.flags;
             *best = p;
             if (best_ms) *best_ms = ms;
@@ -94,6 +95,8 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
     cur.heavy_threshold = 64;
     cur.chunk_arcs = 512;
     cur.lds_depth = 0;
+    cur.compact_ratio = 4;
+    cur.poll_max = 2;
     cur.flags = tc.flags;
     uint32_t n = 0;
     auto measure = [&](const kc_params& p, double* ms) {
@@ -114,19 +117,24 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
         uint32_t kc_params::*field;
         std::vector<uint32_t> values;
     };
-    const std::vector<Axis> axes = {
-        {&kc_params::block, {256, 512, 1024}},
-        {&kc_params::heavy_threshold, {32, 64, 128, 256, 1024}},
-        {&kc_params::chunk_arcs, {256, 512, 1024, 2048}},
-        {&kc_params::lds_depth, {0, 512, 256}},
-    };
     // the variant axis first (an int field), then the unsigned axes
-    for (int v : {KC_VARIANT_WAVE, KC_VARIANT_PREFETCH}) {
+    for (int v : {KC_VARIANT_WAVE, KC_VARIANT_PREFETCH, KC_VARIANT_ASYNC}) {
         kc_params p = cur;
         p.variant = v;
         double ms = 0;
         if ((s = measure(p, &ms)) != KC_OK) return s;
         if (ms < cur_ms) cur = p, cur_ms = ms;
+    }
+    std::vector<Axis> axes = {
+        {&kc_params::block, {256, 512, 1024}},
+        {&kc_params::heavy_threshold, {32, 64, 128, 256, 1024}},
+        {&kc_params::chunk_arcs, {256, 512, 1024, 2048}},
+    };
+    if (cur.variant == KC_VARIANT_ASYNC) {
+        axes.push_back({&kc_params::compact_ratio, {2, 4, 8, 16}});
+        axes.push_back({&kc_params::poll_max, {1, 2, 3}});
+    } else {
+        axes.push_back({&kc_params::lds_depth, {0, 512, 256}});
     }
     for (const Axis& ax : axes) {
         for (uint32_t v : ax.values) {
@@ -141,9 +149,9 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
     *best = cur;
     if (best_ms) *best_ms = cur_ms;
     if (trials) *trials = n;
-    char val[128];
-    std::snprintf(val, sizeof(val), "%d %u %u %u %u %.4f", cur.variant, cur.block, cur.heavy_threshold,
-                  cur.chunk_arcs, cur.lds_depth, cur_ms);
+    char val[160];
+    std::snprintf(val, sizeof(val), "%d %u %u %u %u %u %u %.4f", cur.variant, cur.block, cur.heavy_threshold,
+                  cur.chunk_arcs, cur.lds_depth, cur.compact_ratio, cur.poll_max, cur_ms);
     cache_store(tc.cache_path, key, val);
     return KC_OK;
 }

@@ -185,8 +185,8 @@ def test_tuner_picks_valid_params_and_caches(engine, tmp_path):
     assert t["trials"] >= 10 and t["block"] in (256, 512, 1024)
     t2 = dg.tune(cache_path=cache)
     assert t2["trials"] == 0
-    assert {k: t2[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")} == \
-        {k: t[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")}
-    r = dg.peel(**{k: t[k] for k in ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth")})
+    keys = ("variant", "block", "heavy_threshold", "chunk_arcs", "lds_depth", "compact_ratio", "poll_max")
+    assert {k: t2[k] for k in keys} == {k: t[k] for k in keys}
+    r = dg.peel(**{k: t[k] for k in keys})
     assert (r.core == ref).all() and r.kmax == km
     dg.free()

@@ -188,3 +188,21 @@ def test_serve_survives_idle_longer_than_lease(engine):
         srv.submit(g, req_id=1)
         (c,) = srv.drain(timeout_s=30)
         assert (c["core"] == ref).all()
+
+
+def test_serve_large_request_lane(engine):
+    """Requests above large_arcs are peeled by the single-graph kernel on the large-request lanes,
+    beside the resident kernel that keeps serving the small ones; all results match the oracle."""
+    gs = requests(19, 60, median=30000.0, mmax=600000)
+    ex = expected(gs)
+    with engine.serve(large_arcs=100000, large_lanes=2) as srv:
+        for i, g in enumerate(gs):
+            srv.submit(g, req_id=i)
+        got = {c["req_id"]: c for c in srv.drain(timeout_s=120)}
+        assert srv.pending() == (0, 0, 0)
+    big = [i for i, g in enumerate(gs) if g.arcs >= 100000]
+    assert 5 <= len(big) < len(gs)
+    for i in range(len(gs)):
+        assert got[i]["status"] == 0, i
+        assert (got[i]["core"] == ex[i][0]).all(), i
+        assert got[i]["kmax"] == ex[i][1], i
