@@ -48,6 +48,18 @@ int kcg_write_serialized(const kcg_graph* g, const char* path);
  * the text file and write that cache. *from_cache is set to 1 when the cache was used. */
 int kcg_load_cached(const char* path, const char* outdir, int nthreads, kcg_graph** out, int* from_cache);
 
+/* Binary cache (no reference counterpart; the text cache above stays for interop). File
+ * <outdir>bin-<basename>-<16 hex digits of the source's content hash>.kcg: a 48-byte header
+ * ("KCGBIN\0\1", version 1, V, arcs, source hash, payload hash, reserved) and the raw
+ * little-endian offsets and neighbours. A cache whose header, hashes or offsets do not check out
+ * is refused with KCG_ERR_FORMAT (kcg_load_cached_bin then parses the text and rewrites it). */
+int kcg_hash_file(const char* path, uint64_t* out);
+uint64_t kcg_hash_bytes(const void* p, uint64_t n, uint64_t seed);
+int kcg_write_bincache(const kcg_graph* g, const char* path, uint64_t src_hash);
+/* want_src_hash 0 accepts any source hash. */
+int kcg_read_bincache(const char* path, uint64_t want_src_hash, kcg_graph** out);
+int kcg_load_cached_bin(const char* path, const char* outdir, int nthreads, kcg_graph** out, int* from_cache);
+
 /* Build a CSR from an explicit pair list (same rules as the text loader). */
 int kcg_from_pairs(const uint32_t* src, const uint32_t* dst, uint64_t npairs, int nthreads, kcg_graph** out);
 

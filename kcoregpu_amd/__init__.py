@@ -110,6 +110,28 @@ class Graph:
         return g
 
     @classmethod
+    def load_binary(cls, path: str, output_dir: str = "./output/", nthreads: int = 0) -> "Graph":
+        """Like load(), with the content-keyed binary cache (bin-<basename>-<hash>.kcg)."""
+        out = C.c_void_p()
+        hit = C.c_int(0)
+        st = N.host().kcg_load_cached_bin(os.fsencode(path), os.fsencode(output_dir), nthreads,
+                                          C.byref(out), C.byref(hit))
+        if st != 0:
+            raise KCoreError(st, KCG_ERRORS.get(st, "graph error"))
+        g = Graph(out.value)
+        g.from_cache = bool(hit.value)
+        return g
+
+    @classmethod
+    def read_bincache(cls, path: str, src_hash: int = 0) -> "Graph":
+        return cls._make(N.host().kcg_read_bincache, os.fsencode(path), src_hash)
+
+    def write_bincache(self, path: str, src_hash: int = 0) -> None:
+        st = N.host().kcg_write_bincache(self._h, os.fsencode(path), src_hash)
+        if st:
+            raise KCoreError(st, "write_bincache")
+
+    @classmethod
     def read_serialized(cls, path: str) -> "Graph":
         return cls._make(N.host().kcg_read_serialized, os.fsencode(path))
 
@@ -225,6 +247,19 @@ class Batch:
 
     def vertices(self, g: int) -> slice:
         return slice(int(self.vtx_off[g]), int(self.vtx_off[g + 1]))
+
+
+def hash_file(path: str) -> int:
+    """The binary cache's 64-bit content hash of a file."""
+    h = C.c_uint64()
+    st = N.host().kcg_hash_file(os.fsencode(path), C.byref(h))
+    if st:
+        raise KCoreError(st, "hash_file")
+    return h.value
+
+
+def hash_bytes(data: bytes, seed: int = 0) -> int:
+    return N.host().kcg_hash_bytes(data, len(data), seed)
 
 
 def write_kcore_json(path: str, core: np.ndarray) -> None:

@@ -118,6 +118,12 @@ def host() -> C.CDLL:
                                        C.POINTER(u32p), C.POINTER(u32p)]
         L.kcg_batch_arrays.restype = None
         L.kcg_batch_graph.argtypes = [vp, C.c_uint32, pp]
+        L.kcg_hash_file.argtypes = [C.c_char_p, u64p]
+        L.kcg_hash_bytes.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        L.kcg_hash_bytes.restype = C.c_uint64
+        L.kcg_write_bincache.argtypes = [vp, C.c_char_p, C.c_uint64]
+        L.kcg_read_bincache.argtypes = [C.c_char_p, C.c_uint64, pp]
+        L.kcg_load_cached_bin.argtypes = [C.c_char_p, C.c_char_p, C.c_int, pp, C.POINTER(C.c_int)]
         L.kcg_request_shape.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
                                         C.c_uint64, u64p, C.POINTER(C.c_int)]
         L.kcg_gen_request.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64,

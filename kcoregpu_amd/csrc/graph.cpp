@@ -434,6 +434,32 @@ void Graph::writeKCoreToDisk(std::string file) {
     kcg::write_kcore_json(output_dir + "pkc-kcore-" + file, V, degrees);
 }
 
+bool Graph::loadBinaryCached(std::string input_file) {
+    uint64_t hs = 0;
+    if (kcg::hash_file(input_file, &hs) == kcg::OK &&
+        kcg::read_bincache(kcg::bincache_path(input_file, output_dir, hs), csr_, hs) == kcg::OK) {
+        std::cout << "Reading binary cache... " << std::endl;
+        adopt();
+        return true;
+    }
+    std::cout << "Reading normal file... " << std::endl;
+    auto start = std::chrono::steady_clock::now();
+    int from_cache = 0;
+    const int st = kcg::load_cached_bin(input_file, output_dir, 0, csr_, &from_cache);
+    if (st != kcg::OK) {
+        std::cout << (st == kcg::ERR_IO ? "load graph file failed "
+                                        : "load graph file failed: vertex id or arc count beyond uint32")
+                  << std::endl;
+        V = 0;
+        return false;
+    }
+    adopt();
+    auto end = std::chrono::steady_clock::now();
+    std::cout << "File Loaded in: "
+              << std::chrono::duration_cast<std::chrono::milliseconds>(end - start).count() << std::endl;
+    return true;
+}
+
 Graph::Graph(std::string input_file) {
     if (readSerialized(input_file)) return;
     std::cout << "Reading normal file... " << std::endl;

@@ -6,6 +6,7 @@
 // behaviour (console lines, cache path ./output/serialized-<basename>, JSON writer)
 // follows src/graph.cpp:9-161.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -47,6 +48,14 @@ int write_edgelist(const Csr& g, const std::string& path);
 
 std::string basename_of(const std::string& path);
 
+// ---- binary cache (bincache.cpp) ----
+uint64_t hash_bytes(const void* p, size_t n, uint64_t seed);
+int hash_file(const std::string& path, uint64_t* out);
+int write_bincache(const Csr& g, const std::string& path, uint64_t src_hash);
+int read_bincache(const std::string& path, Csr& g, uint64_t want_src_hash);
+std::string bincache_path(const std::string& path, const std::string& outdir, uint64_t src_hash);
+int load_cached_bin(const std::string& path, const std::string& outdir, int nthreads, Csr& out, int* from_cache);
+
 // ---- generators (generators.cpp) ----
 uint64_t splitmix64(uint64_t x);
 int gen_er(uint32_t n, uint64_t m, uint64_t seed, int nthreads, Csr& out);
@@ -77,6 +86,9 @@ public:
     // Reference readFile exits the process on a missing file; this one returns false after
     // printing the same line so the driver decides.
     bool readFile(std::string input_file);
+    // Binary-cache variant of the constructor: the content-keyed binary cache first, else parse
+    // the text and write that cache. Prints "Reading binary cache... " or the text-load lines.
+    bool loadBinaryCached(std::string input_file);
     void writeKCoreToDisk(std::string file);
     ~Graph();
 

@@ -218,6 +218,40 @@ int kcg_gen_batch(uint32_t G, uint32_t nmin, uint32_t nmax, double dmin, double 
     return pack_batch(ptrs, out);
 }
 
+uint64_t kcg_hash_bytes(const void* p, uint64_t n, uint64_t seed) { return kcg::hash_bytes(p, n, seed); }
+
+int kcg_hash_file(const char* path, uint64_t* out) {
+    if (!path || !out) return KCG_ERR_ARG;
+    return kcg::hash_file(path, out);
+}
+
+int kcg_write_bincache(const kcg_graph* g, const char* path, uint64_t src_hash) {
+    if (!g || !path) return KCG_ERR_ARG;
+    return kcg::write_bincache(g->csr, path, src_hash);
+}
+
+int kcg_read_bincache(const char* path, uint64_t want_src_hash, kcg_graph** out) {
+    if (!path || !out) return KCG_ERR_ARG;
+    kcg::Csr c;
+    const int st = kcg::read_bincache(path, c, want_src_hash);
+    if (st != kcg::OK) return st;
+    auto* g = new kcg_graph;
+    g->csr = std::move(c);
+    *out = g;
+    return KCG_OK;
+}
+
+int kcg_load_cached_bin(const char* path, const char* outdir, int nthreads, kcg_graph** out, int* from_cache) {
+    if (!path || !out) return KCG_ERR_ARG;
+    kcg::Csr c;
+    const int st = kcg::load_cached_bin(path, outdir ? outdir : "./output/", nthreads, c, from_cache);
+    if (st != kcg::OK) return st;
+    auto* g = new kcg_graph;
+    g->csr = std::move(c);
+    *out = g;
+    return KCG_OK;
+}
+
 int kcg_request_shape(uint64_t seed, uint64_t i, double median_m, double sigma, uint64_t mmin, uint64_t mmax,
                       uint64_t* m_out, int* kind_out) {
     if (!m_out || !kind_out || median_m <= 0 || sigma < 0 || mmin == 0 || mmax < mmin) return KCG_ERR_ARG;

@@ -15,11 +15,13 @@
 //   --reps N          repeat each variant N times (the reference uses 1)
 //   --stats           print a JSON stats line per run
 //   --output-dir DIR  directory for the serialized cache and JSON (default ./output/)
+//   --binary-cache    use the content-keyed binary CSR cache instead of the text one
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -55,7 +57,7 @@ int main(int argc, char* argv[]) {
     }
     std::string ds = argv[1];
     std::string core_out, order_out, only;
-    bool json = false, stats = false;
+    bool json = false, stats = false, binary = false;
     int reps = 1;
     for (int i = 2; i < argc; ++i) {
         std::string a = argv[i];
@@ -70,6 +72,7 @@ int main(int argc, char* argv[]) {
         else if (a == "--order-out") order_out = next();
         else if (a == "--json") json = true;
         else if (a == "--stats") stats = true;
+        else if (a == "--binary-cache") binary = true;
         else if (a == "--variant") only = next();
         else if (a == "--reps") reps = std::atoi(next().c_str());
         else if (a == "--output-dir") {
@@ -83,7 +86,14 @@ int main(int argc, char* argv[]) {
     if (reps < 1) reps = 1;
 
     std::cout << "Graph loading Started... " << std::endl;
-    Graph g(ds);
+    std::unique_ptr<Graph> gp;
+    if (binary) {
+        gp.reset(new Graph());
+        gp->loadBinaryCached(ds);
+    } else {
+        gp.reset(new Graph(ds));
+    }
+    Graph& g = *gp;
     if (g.V == 0) return -1;  // the reference exits inside readFile on a missing file
     std::cout << "******************  " << ds << " ****************** " << std::endl;
     std::cout << "V: " << g.V << std::endl;

@@ -63,3 +63,17 @@ def test_driver_lines_cache_and_outputs(tmp_path):
     assert out[3] == f"V: {g.V}" and out[4] == f"E: {g.arcs}"
     assert re.fullmatch(r"async: \d+ ", out[5]) and out[6] == f"Kmax: {kmax}"
     assert (np.fromfile(tmp_path / "core2.u32", np.uint32) == core).all()
+
+
+def test_driver_binary_cache(tmp_path):
+    g = K.Graph.chung_lu(20000, 7)
+    g.write_edgelist(str(tmp_path / "cl.txt"))
+    os.makedirs(tmp_path / "output")
+    core, kmax = O.bz(g.neighbors_offset, g.neighbors)
+    out = run(tmp_path, "cl.txt", "--binary-cache", "--variant", "auto", "--core-out", "c1.u32")
+    assert out[1] == "Reading normal file... " and re.fullmatch(r"File Loaded in: \d+", out[2])
+    out = run(tmp_path, "cl.txt", "--binary-cache", "--variant", "auto", "--core-out", "c2.u32")
+    assert out[1] == "Reading binary cache... "
+    assert out[3] == f"V: {g.V}" and out[4] == f"E: {g.arcs}" and out[6] == f"Kmax: {kmax}"
+    for f in ("c1.u32", "c2.u32"):
+        assert (np.fromfile(tmp_path / f, np.uint32) == core).all()
