@@ -94,7 +94,7 @@ def bench_single(args, K, eng):
                  "compact_ratio": 4, "poll_max": 2, "trials": 0, "ms": None}
     else:
         tt = time.time()
-        tuned = dg.tune(reps=2)
+        tuned = dg.tune(reps=2, cache_path=args.tune_cache)
         log(f"[single] tuned in {time.time() - tt:.1f}s ({tuned['trials']} trials): "
             + " ".join(f"{k}={tuned[k]}" for k in names))
     prm = {k: tuned[k] for k in names}
@@ -166,7 +166,7 @@ def bench_batch(args, K, eng):
     log(f"[batch] {G} graphs, {b.total_vertices} vertices, {b.total_arcs} arcs generated in "
         f"{time.time() - t0:.1f}s")
     db = eng.upload_batch(b)
-    tuned = db.tune(reps=2) if not args.no_tune else {"block": 512, "trials": 0}
+    tuned = db.tune(reps=2, cache_path=args.tune_cache) if not args.no_tune else {"block": 512, "trials": 0}
     log(f"[batch] tuned block {tuned['block']} ({tuned['trials']} trials)")
     flags = K.FLAG_GRAPH
     db.peel(block=tuned["block"], flags=flags, download=False)  # capture + warm
@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: settings tuned once are reused without timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=("single", "batch", "serve"), default=None)
     ap.add_argument("--batch-graphs", type=int, default=4096)
