@@ -35,6 +35,7 @@
 #include <thread>
 #include <vector>
 
+#include "arena.h"
 #include "host_internal.h"
 #include "serve_kernel.h"
 
@@ -61,46 +62,6 @@ struct Req {
     hipEvent_t ev = nullptr;
     int state = 0;  // 0 free, 1 on the device, 2 finishing
     bool large = false;
-};
-
-// first-fit allocator over [0, size)
-struct Arena {
-    std::map<uint64_t, uint64_t> free_;  // offset -> length
-    uint64_t used = 0;
-    void init(uint64_t size) {
-        free_.clear();
-        free_[0] = size;
-        used = 0;
-    }
-    bool alloc(uint64_t n, uint64_t* off) {
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < n) continue;
-            *off = it->first;
-            const uint64_t rest = it->second - n;
-            const uint64_t at = it->first + n;
-            free_.erase(it);
-            if (rest) free_[at] = rest;
-            used += n;
-            return true;
-        }
-        return false;
-    }
-    void release(uint64_t off, uint64_t n) {
-        used -= n;
-        auto it = free_.emplace(off, n).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
 };
 
 template <int BLOCK>
