@@ -94,21 +94,64 @@ __device__ uint32_t wg_peel_graph(BatchState<BLOCK>& S, const uint32_t* off, con
     }
     __syncthreads();
     uint32_t L = 0, levels = 0, rounds = 0;
+    // select reads the degrees as 16-byte device-coherent (SC1) buffer loads, which see the
+    // other waves' atomics; the graph's array may start inside a 16-byte quad (shift words)
+    const uintptr_t cb = reinterpret_cast<uintptr_t>(core);
+    const uint32_t shift = (uint32_t)((cb & 15u) >> 2);
+    const uint64_t nq64 = ((uint64_t)Vg + shift + 3) / 4;
+    const bool quads = nq64 * 16 < (1ull << 31);
+    const uint32_t nq = quads ? (uint32_t)nq64 : 0;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(reinterpret_cast<const void*>(cb & ~(uintptr_t)15), nq * 16u);
     while (true) {
-        // select: every v with core[v] == L (SC1 loads see other waves' atomics)
+        // select: every v with core[v] == L
         uint32_t mn = kSent;
         const uint32_t lstart = S.tail;
-        for (uint32_t i0 = 0; i0 < Vg; i0 += BLOCK * kBU) {
-            uint32_t d[kBU];
+        if (quads) {
+            for (uint32_t q0 = 0; q0 < nq; q0 += BLOCK * kBU) {
+                uint4 d[kBU];
 #pragma unroll
-            for (uint32_t k = 0; k < kBU; ++k) {
-                const uint32_t i = i0 + k * BLOCK + tid;
-                d[k] = i < Vg ? ld_rlx(core + i) : kSent;
+                for (uint32_t k = 0; k < kBU; ++k) {
+                    const uint32_t q = q0 + k * BLOCK + tid;
+                    d[k] = q < nq ? load_quad_coherent(rs, q) : make_uint4(kSent, kSent, kSent, kSent);
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kBU; ++k) {
+                    // vertex of the quad's first word; words before the graph wrap to >= Vg
+                    const uint32_t v0 = 4u * (q0 + k * BLOCK + tid) - shift;
+                    const uint32_t x[4] = {d[k].x, d[k].y, d[k].z, d[k].w};
+                    uint32_t c = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) {
+                        const bool in = v0 + j < Vg;
+                        if (in && x[j] > L && x[j] < mn) mn = x[j];
+                        c += (in && x[j] == L) ? 1u : 0u;
+                    }
+                    // one reservation per wave for all of its matches in this quad row
+                    if (__ballot(c != 0)) {
+                        const uint32_t inc = wave_incl_scan(c);
+                        uint32_t base = 0;
+                        if (lane_id() == 63) base = __hip_atomic_fetch_add(&S.tail, inc, __ATOMIC_RELAXED, KC_WG);
+                        base = (uint32_t)__builtin_amdgcn_readlane((int)base, 63);
+                        uint32_t pos = base + inc - c;
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; ++j)
+                            if (v0 + j < Vg && x[j] == L) st_rlx(order + pos++, v0 + j);
+                    }
+                }
             }
+        } else {
+            for (uint32_t i0 = 0; i0 < Vg; i0 += BLOCK * kBU) {
+                uint32_t d[kBU];
 #pragma unroll
-            for (uint32_t k = 0; k < kBU; ++k) {
-                if (d[k] > L && d[k] < mn) mn = d[k];
-                batch_append(S, order, d[k] == L, i0 + k * BLOCK + tid);
+                for (uint32_t k = 0; k < kBU; ++k) {
+                    const uint32_t i = i0 + k * BLOCK + tid;
+                    d[k] = i < Vg ? ld_rlx(core + i) : kSent;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kBU; ++k) {
+                    if (d[k] > L && d[k] < mn) mn = d[k];
+                    batch_append(S, order, d[k] == L, i0 + k * BLOCK + tid);
+                }
             }
         }
         levels++;
