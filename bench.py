@@ -119,6 +119,13 @@ def bench_single(args, K, eng):
     k_avg_s = float(np.mean(kernel_ms)) / 1e3
     achieved = st["bytes_alg"] / k_avg_s / 1e9
     core = dg.core()
+    # the reference's own shape on the same GPU: host level loop over every level 0..kmax, one
+    # select and per-cascade process launches per level, frontier size read back each time
+    rs = dg.peel("stepwise", flags=K.FLAG_NO_LEVEL_SKIP, download=True)
+    ref_shaped = {"ms": rs.stats["ms_peel"], "levels": rs.stats["levels"], "launches": rs.stats["subrounds"],
+                  "identical": bool((rs.core == core).all()),
+                  "speedup": rs.stats["ms_peel"] / (ms_per_step)}
+    log(f"[single] reference-shaped loop: {rs.stats['ms_peel']:.1f} ms, {rs.stats['subrounds']} launches")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -151,7 +158,8 @@ def bench_single(args, K, eng):
                      "bytes_edge": st["bytes_edge"]},
         "cpu_baseline": cpu,
         "detail": {"ms_peel_mean": float(np.mean(peel_ms)), "ms_kernel_min": float(np.min(kernel_ms)),
-                   "h2d_s": h2d_s, "arcs_per_s": g.arcs / (ms_per_step / 1e3)},
+                   "h2d_s": h2d_s, "arcs_per_s": g.arcs / (ms_per_step / 1e3),
+                   "reference_shaped_gpu": ref_shaped},
     }
     dg.free()
     return out

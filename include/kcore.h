@@ -67,7 +67,9 @@ enum {
     KC_VARIANT_WAVE = 1,     /* one persistent kernel; each wave drains global frontier tickets */
     KC_VARIANT_LDS = 2,      /* + workgroup LDS frontier buffer for cascades, spilling to global */
     KC_VARIANT_PREFETCH = 3, /* + next-batch adjacency prefetch while the current batch runs */
-    KC_VARIANT_STEPWISE = 4, /* reference-shaped: select and process kernels launched per level */
+    KC_VARIANT_STEPWISE = 4, /* reference-shaped: the host runs the level loop, launching a select
+                                kernel and process kernels per level and reading the frontier size
+                                back after each (src/ours.cc:117-160); a baseline, not a fast path */
     KC_VARIANT_ASYNC = 5     /* one persistent kernel; two grid barriers per level: the level's
                                 cascade runs from a shared work queue with no barrier inside it,
                                 and scans read a compacted list once few vertices are left */

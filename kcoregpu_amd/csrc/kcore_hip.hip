@@ -83,6 +83,72 @@ __global__ void selftest_scan_kernel(const uint32_t* in, uint32_t* out) {
     out[threadIdx.x] = wave_incl_scan(in[threadIdx.x]);
 }
 
+// ---- the reference-shaped variant (KC_VARIANT_STEPWISE): the host runs the level loop
+// (src/ours.cc:117-160), launching a select kernel and then process kernels per level and
+// reading the frontier size back after each, as the reference reads its count per level.
+
+// select (src/ours.cc:2-33): append every v with core[v] == L to order[] at ctr[0]; ctr[1] gets
+// the smallest degree above L. Plain loads: nothing changes core[] during this launch.
+__global__ void step_select(const uint32_t* core, uint32_t V, uint32_t L, uint32_t* order, uint32_t* ctr) {
+    uint32_t mn = kcdev::kSent;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < V; b += stride) {
+        const uint32_t v = b + threadIdx.x;
+        const uint32_t d = v < V ? core[v] : kcdev::kSent;
+        if (d > L && d < mn) mn = d;
+        const uint64_t m = __ballot(v < V && d == L);
+        if (m) {
+            uint32_t base = 0;
+            if (kcdev::lane_id() == 0) base = kcdev::add_rlx(ctr, (uint32_t)__popcll(m));
+            base = kcdev::bcast_first(base);
+            if (v < V && d == L) order[base + kcdev::rank_below(m)] = v;
+        }
+    }
+    mn = kcdev::wave_min(mn);
+    if (kcdev::lane_id() == 0 && mn != kcdev::kSent) __hip_atomic_fetch_min(ctr + 1, mn, __ATOMIC_RELAXED, KC_AGENT);
+}
+
+// process (src/ours.cc:38-114): one wave per frontier vertex of order[beg, end); its lanes walk
+// the adjacency; u with core[u] > L is decremented, a == L+1 appends u at ctr[0], a <= L undoes.
+__global__ void step_process(const uint32_t* off, const uint32_t* nbr, uint32_t* core, uint32_t* order,
+                            uint32_t beg, uint32_t end, uint32_t L, uint32_t* ctr) {
+    const uint32_t lane = kcdev::lane_id();
+    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t GW = gridDim.x * blockDim.x / 64;
+    uint32_t mn = kcdev::kSent;
+    for (uint32_t i = beg + gw; i < end; i += GW) {
+        const uint32_t v = order[i];
+        const uint32_t s = off[v], e = off[v + 1];
+        for (uint32_t j0 = s; j0 < e; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            bool cross = false;
+            uint32_t u = 0;
+            if (j < e) {
+                u = nbr[j];
+                if (core[u] > L) {
+                    const uint32_t a = __hip_atomic_fetch_sub(core + u, 1u, __ATOMIC_RELAXED, KC_AGENT);
+                    if (a == L + 1) {
+                        cross = true;
+                    } else if (a <= L) {
+                        __hip_atomic_fetch_add(core + u, 1u, __ATOMIC_RELAXED, KC_AGENT);
+                    } else if (a - 1 < mn) {
+                        mn = a - 1;
+                    }
+                }
+            }
+            const uint64_t m = __ballot(cross);
+            if (m) {
+                uint32_t base = 0;
+                if (lane == 0) base = kcdev::add_rlx(ctr, (uint32_t)__popcll(m));
+                base = kcdev::bcast_first(base);
+                if (cross) order[base + kcdev::rank_below(m)] = u;
+            }
+        }
+    }
+    mn = kcdev::wave_min(mn);
+    if (lane == 0 && mn != kcdev::kSent) __hip_atomic_fetch_min(ctr + 1, mn, __ATOMIC_RELAXED, KC_AGENT);
+}
+
 constexpr uint32_t kTraceCap = 1u << 16;
 constexpr uint32_t kHeavyMin = 16;   // smallest degree that may be processed as chunks
 constexpr uint32_t kChunkMin = 64;   // smallest chunk
@@ -148,6 +214,74 @@ static kc_status launch_batch(kc_ctx* c, const BatchArgs& a) {
     uint32_t grid = std::min<uint32_t>((uint32_t)per_cu * (uint32_t)c->cus, a.G);
     peel_batch<BLOCK><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
     KC_TRY(hipGetLastError());
+    return KC_OK;
+}
+
+// The reference's host-driven level loop over step_select / step_process (see above). Returns
+// with core[] holding the core numbers; order[] / shell[] as for the other variants.
+static kc_status peel_stepwise(kc_ctx* c, kc_graph* g, uint32_t flags, uint32_t* kmax_out, uint32_t* levels_out,
+                               uint32_t* launches_out, float* ms_out) {
+    hipStream_t st = c->stream;
+    uint32_t* ctr = g->d_ctrl;  // [0] order tail, [1] smallest degree above the level
+    uint32_t* h = nullptr;
+    KC_TRY(hipHostMalloc(&h, 64, hipHostMallocDefault));
+    std::vector<uint32_t> shell(g->dmax + 3, 0);
+    const uint32_t grid = std::min<uint32_t>(4096u, std::max<uint32_t>(1u, (g->V + 255) / 256));
+    const uint32_t pgrid = (uint32_t)c->cus * 8;
+    kc_status rs = KC_OK;
+    uint32_t L = 0, levels = 0, launches = 0, tail = 0;
+    KC_TRY(hipEventRecord(c->ev0, st));
+    fill_kernel<uint32_t><<<1, 64, 0, st>>>(ctr, 0u, 64);
+    copy_kernel<<<2048, 256, 0, st>>>(reinterpret_cast<const uint4*>(g->d_deg0), reinterpret_cast<uint4*>(g->d_core),
+                                      g->padV / 4);
+    while (true) {
+        shell[L] = tail;
+        fill_kernel<uint32_t><<<1, 64, 0, st>>>(ctr + 1, kcdev::kSent, 1);
+        step_select<<<grid, 256, 0, st>>>(g->d_core, g->V, L, g->d_order, ctr);
+        ++launches;
+        ++levels;
+        uint32_t beg = tail;
+        if (hipMemcpyAsync(h, ctr, 8, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            rs = KC_ERR_HIP;
+            break;
+        }
+        uint32_t end = h[0];
+        while (beg < end) {
+            step_process<<<pgrid, 256, 0, st>>>(g->d_off, g->d_nbr, g->d_core, g->d_order, beg, end, L, ctr);
+            ++launches;
+            if (hipMemcpyAsync(h, ctr, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                rs = KC_ERR_HIP;
+                break;
+            }
+            beg = end;
+            end = h[0];
+        }
+        if (rs != KC_OK) break;
+        tail = end;
+        if (tail >= g->V) {
+            shell[L + 1] = tail;
+            break;
+        }
+        uint32_t next = L + 1;
+        if (!(flags & KC_FLAG_NO_LEVEL_SKIP) && h[1] != kcdev::kSent && h[1] > next) next = h[1];
+        if (next > g->dmax) {
+            rs = KC_ERR_NO_PROGRESS;
+            break;
+        }
+        for (uint32_t k = L + 1; k <= next; ++k) shell[k] = tail;
+        L = next;
+    }
+    KC_TRY(hipEventRecord(c->ev1, st));
+    KC_TRY(hipEventSynchronize(c->ev1));
+    (void)hipHostFree(h);
+    if (rs != KC_OK) return rs;
+    KC_TRY(hipGetLastError());
+    KC_TRY(hipMemcpy(g->d_shell, shell.data(), (size_t)(L + 2) * 4, hipMemcpyHostToDevice));
+    KC_TRY(hipEventElapsedTime(ms_out, c->ev0, c->ev1));
+    *kmax_out = L;
+    *levels_out = levels;
+    *launches_out = launches;
     return KC_OK;
 }
 
@@ -302,8 +436,32 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
     kc_params prm{};
     if (p) prm = *p;
     int variant = prm.variant ? prm.variant : KC_VARIANT_LDS;
-    if (variant != KC_VARIANT_ASYNC && (variant < KC_VARIANT_WAVE || variant > KC_VARIANT_PREFETCH))
-        return KC_ERR_ARG;
+    if (variant < KC_VARIANT_WAVE || variant > KC_VARIANT_ASYNC) return KC_ERR_ARG;
+    if (variant == KC_VARIANT_STEPWISE) {
+        uint32_t kmax = 0, levels = 0, launches = 0;
+        float ms = 0;
+        const kc_status s = peel_stepwise(c, g, prm.flags, &kmax, &levels, &launches, &ms);
+        if (s != KC_OK) return s;
+        g->last_kmax = kmax;
+        g->last_ntrace = 0;
+        g->peeled = true;
+        if (core_out) KC_TRY(hipMemcpy(core_out, g->d_core, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+        if (kmax_out) *kmax_out = kmax;
+        if (st) {
+            memset(st, 0, sizeof(*st));
+            st->ms_peel = ms;
+            st->ms_kernel = ms;
+            st->kmax = kmax;
+            st->levels = levels;
+            st->subrounds = launches;
+            st->scanned_entries = (uint64_t)levels * g->V;
+            st->bytes_scan = 4 * st->scanned_entries;
+            st->bytes_edge = 16ull * g->V + 20ull * (g->arcs / 2);
+            st->bytes_alg = st->bytes_scan + st->bytes_edge;
+            st->block = 256;
+        }
+        return KC_OK;
+    }
     const bool async = variant == KC_VARIANT_ASYNC;
     if (async && !g->d_alive) KC_TRY(hipMalloc(&g->d_alive, (size_t)g->padV * 8));
     const uint32_t block = prm.block ? prm.block : 512;

@@ -11,7 +11,8 @@
 //   --core-out PATH   write the core numbers as raw little-endian uint32[V]
 //   --json            write ./output/pkc-kcore-<edgelist> like Graph::writeKCoreToDisk
 //   --order-out PATH  write the peel order (uint32[V]) and PATH.shells (k-shell starts)
-//   --variant NAME    run only one variant: wave | lds | prefetch | async | auto | stepwise
+//   --variant NAME    run only one variant: wave | lds | prefetch | async | auto | stepwise (the
+//                     reference-shaped host level loop)
 //   --reps N          repeat each variant N times (the reference uses 1)
 //   --stats           print a JSON stats line per run
 //   --output-dir DIR  directory for the serialized cache and JSON (default ./output/)

@@ -15,7 +15,7 @@ from conftest import closed_form_graphs, csr_from_edges
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-VARIANTS = ["wave", "lds", "prefetch", "async"]
+VARIANTS = ["wave", "lds", "prefetch", "async", "stepwise"]
 
 
 def say(*a):
