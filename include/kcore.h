@@ -92,9 +92,11 @@ typedef struct {
 enum {
     KC_FLAG_NO_LEVEL_SKIP = 1u << 0, /* scan every level 0..kmax like the reference, even empty ones */
     KC_FLAG_TRACE = 1u << 8,         /* record a device timestamp at every grid barrier */
-    KC_FLAG_GRAPH = 1u << 9          /* kc_batch_peel: launch the whole peel (state reset, degree
+    KC_FLAG_GRAPH = 1u << 9,         /* kc_batch_peel: launch the whole peel (state reset, degree
                                         copy, persistent kernel) as one HIP graph, captured on first
                                         use for each (block, flags) and replayed afterwards */
+    KC_FLAG_DIRECT_POLL = 1u << 10   /* ASYNC: idle workgroups poll the shared queue counters
+                                        directly instead of their XCD group's work hint */
 };
 
 typedef struct {

@@ -29,6 +29,7 @@ VARIANTS = {"auto": 0, "wave": 1, "lds": 2, "prefetch": 3, "stepwise": 4, "async
 FLAG_NO_LEVEL_SKIP = 1
 FLAG_TRACE = 1 << 8
 FLAG_GRAPH = 1 << 9
+FLAG_DIRECT_POLL = 1 << 10
 
 KCG_ERRORS = {1: "I/O error", 2: "vertex id or arc count beyond uint32", 3: "bad argument",
               4: "malformed serialized cache"}

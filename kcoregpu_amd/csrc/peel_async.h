@@ -50,7 +50,8 @@ enum : uint32_t {
     A_SVEND,            // statically over every wave before the queues open
     A_SCBEG,
     A_SCEND,
-    A_LDONE,            // kXcds level-done flags
+    A_HINT,             // kXcds work hints: advanced after a workgroup appended crossings
+    A_LDONE = A_HINT + kXcds,  // kXcds level-done flags
     A_COUNT = A_LDONE + kXcds
 };
 
@@ -174,6 +175,16 @@ __device__ __forceinline__ void signal_level_done(const PeelArgs& a, uint32_t ge
     for (uint32_t x = 0; x < kXcds; ++x) st_rlx(cw(a, A_LDONE + x), gen);
 }
 
+// After a workgroup's appends are complete: wake the idle workgroups of every XCD group, which
+// poll their group's hint instead of the shared queue counters.
+template <int BLOCK>
+__device__ __forceinline__ void publish_hint(const PeelArgs& a, WgStateA<BLOCK>& S) {
+    if (S.appended) {
+        S.appended = 0;
+        for (uint32_t x = 0; x < kXcds; ++x) add_rlx(cw(a, A_HINT + x), 1u);
+    }
+}
+
 __device__ __forceinline__ void idle_sleep(uint32_t nap, uint32_t poll_max) {
     if (nap < 8 || poll_max == 0) {
         __builtin_amdgcn_s_sleep(2);
@@ -205,6 +216,7 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
         if (threadIdx.x == 0) {
             S.vcount = 0;
             S.ccount = 0;
+            S.appended = 0;
         }
         __syncthreads();
         uint32_t nc = 0, nv = 0;
@@ -228,6 +240,7 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
             __hip_atomic_fetch_add(&S.ccount, nc, __ATOMIC_RELAXED, KC_WG);
         }
         __syncthreads();
+        if (threadIdx.x == 0) publish_hint(a, S);
         if (threadIdx.x == 0 && (S.vcount | S.ccount)) {
             if (S.vcount) add_rlx(cw(a, A_VDONE), S.vcount);
             if (S.ccount) add_rlx(cw(a, A_CDONE), S.ccount);
@@ -240,34 +253,48 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
         if (threadIdx.x == 0) {
             uint32_t kind = 3, beg = 0, n = 0, nap = 0;
             Waiter w;
+            uint32_t* hint = cw(a, A_HINT + S.vbid % kXcds);
+            const bool direct = (a.flags & FLAG_DIRECT_POLL) != 0;
+            uint32_t seen = direct ? 0 : ld_rlx(hint);
+            bool check = true;
             while (true) {
-                const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
-                const uint32_t ch = ld_rlx(cw(a, A_CHEAD));
-                if (ch < ct) {
-                    const uint32_t k = min(WPB, ct - ch);
-                    if (cas_rlx(cw(a, A_CHEAD), ch, ch + k)) {
-                        kind = 1, beg = ch, n = k;
-                        break;
+                if (check) {
+                    const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
+                    const uint32_t ch = ld_rlx(cw(a, A_CHEAD));
+                    if (ch < ct) {
+                        const uint32_t k = min(WPB, ct - ch);
+                        if (cas_rlx(cw(a, A_CHEAD), ch, ch + k)) {
+                            kind = 1, beg = ch, n = k;
+                            break;
+                        }
+                        continue;
                     }
-                    continue;
-                }
-                const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
-                const uint32_t vh = ld_rlx(cw(a, A_VHEAD));
-                if (vh < vt) {
-                    const uint32_t k = min((uint32_t)BLOCK, vt - vh);
-                    if (cas_rlx(cw(a, A_VHEAD), vh, vh + k)) {
-                        kind = 2, beg = vh, n = k;
-                        break;
+                    const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
+                    const uint32_t vh = ld_rlx(cw(a, A_VHEAD));
+                    if (vh < vt) {
+                        const uint32_t k = min((uint32_t)BLOCK, vt - vh);
+                        if (cas_rlx(cw(a, A_VHEAD), vh, vh + k)) {
+                            kind = 2, beg = vh, n = k;
+                            break;
+                        }
+                        continue;
                     }
-                    continue;
                 }
                 if (ld_rlx(ldone) == gen) break;
-                if ((++nap & 7) == 0 && level_complete(a)) {
+                if ((++nap & 31) == 0 && level_complete(a)) {
                     signal_level_done(a, gen);
                     break;
                 }
                 if (w.expired(cw(a, C_ERR), a.time_limit)) break;
                 idle_sleep(nap, a.poll_max);
+                if (direct) {
+                    check = true;
+                } else {
+                    // full check when the group's hint moved, and every 64 polls regardless
+                    const uint32_t h = ld_rlx(hint);
+                    check = h != seen || (nap & 63) == 0;
+                    seen = h;
+                }
             }
             S.kind = kind;
             S.beg = beg;
@@ -308,6 +335,7 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
         drain_memory();
         __syncthreads();
         if (threadIdx.x == 0) {
+            publish_hint(a, S);
             add_rlx(cw(a, kind == 1 ? A_CDONE : A_VDONE), n);
             order_point();
             if (level_complete(a)) signal_level_done(a, gen);

@@ -108,6 +108,7 @@ struct PeelArgs {
 
 constexpr uint32_t FLAG_NO_LEVEL_SKIP = 1u;
 constexpr uint32_t FLAG_TRACE = 1u << 8;
+constexpr uint32_t FLAG_DIRECT_POLL = 1u << 10;
 
 __device__ __forceinline__ uint32_t* cw(const PeelArgs& a, uint32_t slot) { return a.ctrl + slot * kStride; }
 
@@ -127,6 +128,7 @@ struct WgState {
     uint32_t gen;    // barrier generation read on arrival
     uint32_t vbid;   // this workgroup's index among the participants
     uint32_t npart;  // participating workgroups
+    uint32_t appended;  // cascade-queue kernel: a wave appended crossings since the last hint
 };
 
 constexpr uint32_t kClosed = 0x80000000u;
@@ -271,6 +273,7 @@ __device__ void flush_stage(const PeelArgs& a, WgState<BLOCK>& S, uint32_t& n, b
     const uint32_t* st = S.stage[threadIdx.x / 64];
     const uint32_t rb = wave_reserve(cw(a, C_OTAIL), n);
     for (uint32_t i = lane; i < n; i += 64) st_rlx(a.order + rb + i, st[i]);
+    if (a.async_mode && crossings && lane == 0) S.appended = 1;
     for (uint32_t i0 = 0; i0 < n; i0 += 64) {
         const bool has = i0 + lane < n;
         const uint32_t u = has ? st[i0 + lane] : 0;

@@ -36,6 +36,7 @@ def test_async_er_settings(engine):
         check(g, dg, core, kmax, compact_ratio=ratio)
     for pm in (1, 2, 3):
         check(g, dg, core, kmax, poll_max=pm)
+    check(g, dg, core, kmax, flags=K.FLAG_DIRECT_POLL)
     for block in (256, 512, 1024):
         check(g, dg, core, kmax, block=block)
     r = check(g, dg, core, kmax, flags=K.FLAG_NO_LEVEL_SKIP)
