@@ -121,7 +121,9 @@ const char* kc_status_str(kc_status s);
 kc_status kc_open(int device, kc_ctx** out);
 void kc_close(kc_ctx* ctx);
 
-/* Copies the CSR to the device and allocates the peel state for it. */
+/* Copies the CSR to the device, checks it there (offsets monotone from 0 to arcs, neighbour ids
+ * < V: KC_ERR_ARG otherwise) and allocates the peel state for it. KC_ERR_CAPACITY when
+ * V > 2^30 - 256 or arcs >= 2^32. */
 kc_status kc_graph_upload(kc_ctx* ctx, const kc_csr* g, kc_graph** out);
 void kc_graph_free(kc_graph* g);
 void kc_graph_info(const kc_graph* g, uint32_t* V, uint64_t* arcs, uint32_t* dmax);

@@ -356,7 +356,8 @@ void kc_graph_free(kc_graph* g) {
 kc_status kc_graph_upload(kc_ctx* c, const kc_csr* h, kc_graph** out) {
     if (!c || !h || !out || !h->offsets || (h->arcs && !h->neighbors)) return KC_ERR_ARG;
     // V < 2^30: the degree array must stay addressable by one buffer descriptor (< 4 GiB)
-    if (h->V == 0 || h->V >= (1u << 30) || h->arcs >= 0xFFFFFFFFull) return KC_ERR_CAPACITY;
+    // padV = V rounded up to 256 must keep the degree array below 4 GiB (one buffer descriptor)
+    if (h->V == 0 || h->V > (1u << 30) - 256 || h->arcs >= 0xFFFFFFFFull) return KC_ERR_CAPACITY;
     if (h->offsets[0] != 0 || h->offsets[h->V] != h->arcs) return KC_ERR_ARG;
     KC_TRY(hipSetDevice(c->device));
     kc_graph* g = new (std::nothrow) kc_graph;

@@ -117,12 +117,18 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
         uint32_t kc_params::*field;
         std::vector<uint32_t> values;
     };
+    // A candidate that fails with a device-side status (a wait past its time limit, the
+    // no-progress guard) is skipped; a HIP runtime failure ends the tuning.
+    auto fatal = [](kc_status st) { return st != KC_OK && st != KC_ERR_DEVICE && st != KC_ERR_NO_PROGRESS; };
     // the variant axis first (an int field), then the unsigned axes
     for (int v : {KC_VARIANT_WAVE, KC_VARIANT_PREFETCH, KC_VARIANT_ASYNC}) {
         kc_params p = cur;
         p.variant = v;
         double ms = 0;
-        if ((s = measure(p, &ms)) != KC_OK) return s;
+        if ((s = measure(p, &ms)) != KC_OK) {
+            if (fatal(s)) return s;
+            continue;
+        }
         if (ms < cur_ms) cur = p, cur_ms = ms;
     }
     std::vector<Axis> axes = {
@@ -142,7 +148,10 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
             kc_params p = cur;
             p.*ax.field = v;
             double ms = 0;
-            if ((s = measure(p, &ms)) != KC_OK) return s;
+            if ((s = measure(p, &ms)) != KC_OK) {
+                if (fatal(s)) return s;
+                continue;
+            }
             if (ms < cur_ms) cur = p, cur_ms = ms;
         }
     }
