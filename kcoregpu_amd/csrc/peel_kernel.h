@@ -286,7 +286,8 @@ __device__ void flush_stage(const PeelArgs& a, WgState<BLOCK>& S, uint32_t& n, b
         if (local && __ballot(light)) {
             const uint32_t incl = wave_incl_scan(light ? d : 0);
             uint32_t abase = 0;
-            if (lane == 0) abase = __hip_atomic_fetch_add(&S.larcs, read_lane(incl, 63), __ATOMIC_RELAXED, KC_WG);
+            const uint32_t wave_arcs = read_lane(incl, 63);
+            if (lane == 0) abase = __hip_atomic_fetch_add(&S.larcs, wave_arcs, __ATOMIC_RELAXED, KC_WG);
             abase = bcast_first(abase);
             const bool fits = light && abase + incl <= a.lq_arcs;
             const uint64_t fm = __ballot(fits);

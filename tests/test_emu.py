@@ -1,0 +1,176 @@
+"""The device kernels run on the CPU through the wave64 emulator (tests/emu): the same headers the
+GPU build compiles, with every wave operation, workgroup barrier and atomic emulated, several
+workgroups running at once. Results must equal the oracle's. This checks the kernels' logic —
+indexing, queue protocols, termination, level skipping, compaction, validation — without a GPU;
+it cannot check the GPU memory model (the host is more strongly ordered)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import kcoregpu_amd as K
+import oracle as O
+from conftest import closed_form_graphs, csr_from_edges
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLANG = "/opt/rocm/llvm/bin/clang++"
+pytestmark = pytest.mark.skipif(not os.path.exists(CLANG) and shutil.which("clang++") is None,
+                                reason="needs clang++ (ext_vector_type)")
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("emu") / "emu"
+    cc = CLANG if os.path.exists(CLANG) else "clang++"
+    subprocess.run([cc, "-O1", "-std=c++17", "-pthread", "-I", os.path.join(HERE, "emu"), "-o", str(exe),
+                    os.path.join(HERE, "emu", "emu_main.cpp")], check=True)
+    return str(exe)
+
+
+def write_graph(path, off, nbr):
+    with open(path, "wb") as f:
+        f.write(np.uint32(len(off) - 1).tobytes())
+        f.write(np.uint64(len(nbr)).tobytes())
+        f.write(np.ascontiguousarray(off, np.uint32).tobytes())
+        f.write(np.ascontiguousarray(nbr, np.uint32).tobytes())
+
+
+def run(emu, *args):
+    r = subprocess.run([emu, *map(str, args)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+def peel(emu, tmp_path, off, nbr, variant, *extra):
+    gp, op = tmp_path / "g.bin", tmp_path / "o.bin"
+    write_graph(gp, off, nbr)
+    info = run(emu, "peel", variant, gp, op, *extra)
+    d = np.fromfile(op, np.uint32)
+    V = len(off) - 1
+    kmax, err = int(d[0]), int(d[1])
+    core, order, shells = d[2:2 + V], d[2 + V:2 + 2 * V], d[2 + 2 * V:]
+    return kmax, err, core, order, shells, info
+
+
+def check(off, nbr, kmax, err, core, order, shells):
+    ref, km = O.bz(off, nbr)
+    assert err == 0
+    assert kmax == km
+    assert (core == ref).all(), int(np.flatnonzero(core != ref)[0])
+    assert O.check_certificate(off, nbr, core, order) == 0
+    assert shells[0] == 0 and shells[kmax + 1] == len(off) - 1
+
+
+def graphs():
+    out = []
+    for name, n, edges, _ in closed_form_graphs():
+        off, nbr = csr_from_edges(n, edges)
+        out.append((name, off, nbr))
+    for seed, (n, m) in enumerate([(3000, 15000), (5000, 60000)]):
+        g = K.Graph.erdos_renyi(n, m, seed + 1)
+        out.append((f"er{n}", g.neighbors_offset.copy(), g.neighbors.copy()))
+    g = K.Graph.chung_lu(8000, 3)
+    out.append(("chung_lu8000", g.neighbors_offset.copy(), g.neighbors.copy()))
+    g = K.Graph.rmat(12, 5)
+    out.append(("rmat12", g.neighbors_offset.copy(), g.neighbors.copy()))
+    # a hub of degree 3000 inside a 9-clique: chunked arcs
+    src = [0] * 3000 + [i for i in range(1, 9) for j in range(i + 1, 10)] + [0] * 9
+    dst = list(range(10, 3010)) + [j for i in range(1, 9) for j in range(i + 1, 10)] + list(range(1, 10))
+    g = K.Graph.from_pairs(np.array(src), np.array(dst))
+    out.append(("hub3000", g.neighbors_offset.copy(), g.neighbors.copy()))
+    return out
+
+
+GRAPHS = graphs()
+
+
+@pytest.mark.parametrize("variant", ["persistent", "async"])
+@pytest.mark.parametrize("case", GRAPHS, ids=[c[0] for c in GRAPHS])
+def test_single_graph_kernels(emu, tmp_path, variant, case):
+    name, off, nbr = case
+    check(off, nbr, *peel(emu, tmp_path, off, nbr, variant)[:5])
+
+
+@pytest.mark.parametrize("settings", [
+    dict(heavy=16, chunk=64, flags=0, grid=3, ratio=4),
+    dict(heavy=64, chunk=512, flags=1, grid=4, ratio=4),      # reference level loop
+    dict(heavy=64, chunk=512, flags=0, grid=2, ratio=1),      # compact at every chance
+    dict(heavy=64, chunk=512, flags=0, grid=5, ratio=64),
+    dict(heavy=0xFFFFFFFF, chunk=512, flags=1024, grid=4, ratio=4),  # never chunk; direct polling
+])
+def test_async_settings(emu, tmp_path, settings):
+    g = K.Graph.rmat(13, 9)
+    off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
+    res = peel(emu, tmp_path, off, nbr, "async", settings["heavy"], settings["chunk"], settings["flags"],
+               settings["grid"], settings["ratio"])
+    check(off, nbr, *res[:5])
+    info = dict(zip(res[5].split()[0::2], map(int, res[5].split()[1::2])))
+    if settings["flags"] & 1:
+        assert info["levels"] == res[0] + 1
+    if settings["ratio"] == 1:
+        assert info["list"] > 0 and info["surv"] > 0  # the alive list was built and scanned
+
+
+def test_prefetch_variant(emu, tmp_path):
+    g = K.Graph.chung_lu(6000, 8)
+    off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
+    check(off, nbr, *peel(emu, tmp_path, off, nbr, "prefetch")[:5])
+
+
+def test_batch_kernel(emu, tmp_path):
+    paths, refs = [], []
+    cases = [c for c in GRAPHS if c[0] in ("K5", "cycle7", "star15", "isolated", "er3000", "chung_lu8000",
+                                            "hub3000", "rmat12", "grid3x4", "union")]
+    # graph arrays land at every alignment inside the packed arrays (quad-load shifts)
+    for i, (name, off, nbr) in enumerate(cases):
+        p = tmp_path / f"b{i}.bin"
+        write_graph(p, off, nbr)
+        paths.append(p)
+        refs.append(O.bz(off, nbr))
+    out = tmp_path / "bo.bin"
+    run(emu, "batch", *paths, out)
+    d = np.fromfile(out, np.uint32)
+    pos = 0
+    for (name, off, nbr), (ref, km) in zip(cases, refs):
+        V = len(off) - 1
+        assert d[pos] == km and d[pos + 1] == 0, name
+        assert (d[pos + 2:pos + 2 + V] == ref).all(), name
+        pos += 2 + V
+
+
+def test_serve_kernel(emu, tmp_path):
+    cases = [c for c in GRAPHS if c[0] in ("K12", "er3000", "rmat12", "hub3000", "isolated", "cycle16",
+                                            "chung_lu8000", "K3,5", "path9")]
+    paths, refs = [], []
+    for i, (name, off, nbr) in enumerate(cases):
+        p = tmp_path / f"s{i}.bin"
+        write_graph(p, off, nbr)
+        paths.append(p)
+        refs.append(O.bz(off, nbr))
+    # one malformed request: a neighbour id equal to V
+    off, nbr = cases[1][1], cases[1][2].copy()
+    nbr[7] = len(off) - 1
+    bad = tmp_path / "bad.bin"
+    write_graph(bad, off, nbr)
+    out = tmp_path / "so.bin"
+    run(emu, "serve", *paths[:4], bad, *paths[4:], out)
+    d = np.fromfile(out, np.uint32)
+    pos = 0
+    order = list(range(4)) + [None] + list(range(4, len(cases)))
+    for idx in order:
+        if idx is None:
+            V = len(off) - 1
+            assert d[pos + 1] == 2  # SERVE_BAD_CSR, never peeled
+            pos += 2 + V
+            continue
+        (name, o, n), (ref, km) = cases[idx], refs[idx]
+        V = len(o) - 1
+        assert d[pos] == km and d[pos + 1] == 0, name
+        assert (d[pos + 2:pos + 2 + V] == ref).all(), name
+        pos += 2 + V
+
+
+def test_serve_lease(emu):
+    assert "expired flag 1" in run(emu, "lease")
