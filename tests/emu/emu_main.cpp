@@ -4,6 +4,7 @@
 // argument set-up mirrors what kcore_hip.hip and serve.hip do on the GPU.
 //
 //   emu peel <variant: persistent|prefetch|async> <graph> <out> [heavy chunk flags grid ratio]
+//   emu stepwise <graph> <out> [flags]  (the host level loop of kcore_hip.hip's peel_stepwise)
 //   emu batch <graphs...> <out>
 //   emu serve <graphs...> <out>          (every request published before the kernel starts)
 //   emu lease                            (an idle serving kernel leaves when the heartbeat stops)
@@ -20,6 +21,7 @@
 #include "../../kcoregpu_amd/csrc/batch_kernel.h"
 #include "../../kcoregpu_amd/csrc/peel_async.h"
 #include "../../kcoregpu_amd/csrc/serve_kernel.h"
+#include "../../kcoregpu_amd/csrc/stepwise_kernel.h"
 
 using namespace kcdev;
 
@@ -124,6 +126,56 @@ int run_peel(int argc, char** argv) {
     std::fclose(f);
     std::printf("levels %u subrounds %u participants %u dense %u list %u surv %u chunks %u\n", r[R_LEVELS],
                 r[R_SUBROUNDS], r[R_NPART], r[R_WORDS], r[R_WORDS + 1], r[R_WORDS + 2], r[R_WORDS + 3]);
+    return 0;
+}
+
+int run_stepwise(int argc, char** argv) {
+    G g = read_graph(argv[2]);
+    const uint32_t flags = argc > 4 ? (uint32_t)std::strtoul(argv[4], nullptr, 10) : 0;
+    const uint32_t V = g.V;
+    uint32_t dmax = 0;
+    for (uint32_t v = 0; v < V; ++v) dmax = std::max(dmax, g.off[v + 1] - g.off[v]);
+    uint32_t* core = zalloc<uint32_t>(V);
+    for (uint32_t v = 0; v < V; ++v) core[v] = g.off[v + 1] - g.off[v];
+    uint32_t* order = zalloc<uint32_t>(V);
+    uint32_t* ctr = zalloc<uint32_t>(64);
+    std::vector<uint32_t> shell(dmax + 3, 0);
+    uint32_t L = 0, tail = 0, levels = 0, launches = 0, status = 0;
+    while (true) {
+        shell[L] = tail;
+        ctr[1] = kSent;
+        kcemu::launch(2, 256, [&] { step_select(core, V, L, order, ctr); });
+        ++launches;
+        ++levels;
+        uint32_t beg = tail, end = ctr[0];
+        while (beg < end) {
+            kcemu::launch(3, 256, [&] { step_process(g.off.data(), g.nbr.data(), core, order, beg, end, L, ctr); });
+            ++launches;
+            beg = end;
+            end = ctr[0];
+        }
+        tail = end;
+        if (tail >= V) {
+            shell[L + 1] = tail;
+            break;
+        }
+        uint32_t next = L + 1;
+        if (!(flags & 1u) && ctr[1] != kSent && ctr[1] > next) next = ctr[1];
+        if (next > dmax) {
+            status = 2;
+            break;
+        }
+        for (uint32_t k = L + 1; k <= next; ++k) shell[k] = tail;
+        L = next;
+    }
+    FILE* f = std::fopen(argv[3], "wb");
+    uint32_t head[2] = {L, status};
+    std::fwrite(head, 4, 2, f);
+    std::fwrite(core, 4, V, f);
+    std::fwrite(order, 4, V, f);
+    std::fwrite(shell.data(), 4, L + 2, f);
+    std::fclose(f);
+    std::printf("levels %u launches %u\n", levels, launches);
     return 0;
 }
 
@@ -279,6 +331,7 @@ int main(int argc, char** argv) {
     const std::string cmd = argv[1];
     if (cmd == "peel" && argc >= 5) return run_peel(argc, argv);
     if (cmd == "batch" && argc >= 4) return run_batch(argc, argv);
+    if (cmd == "stepwise" && argc >= 4) return run_stepwise(argc, argv);
     if (cmd == "serve" && argc >= 4) return run_serve(argc, argv);
     if (cmd == "lease") return run_lease();
     return 2;

@@ -113,6 +113,20 @@ def test_async_settings(emu, tmp_path, settings):
         assert info["list"] > 0 and info["surv"] > 0  # the alive list was built and scanned
 
 
+@pytest.mark.parametrize("flags", [0, 1])
+@pytest.mark.parametrize("name", ["K12", "union", "er3000", "rmat12", "hub3000", "isolated"])
+def test_stepwise_kernels(emu, tmp_path, name, flags):
+    _, off, nbr = next(c for c in GRAPHS if c[0] == name)
+    gp, op = tmp_path / "g.bin", tmp_path / "o.bin"
+    write_graph(gp, off, nbr)
+    info = run(emu, "stepwise", gp, op, flags)
+    d = np.fromfile(op, np.uint32)
+    V = len(off) - 1
+    check(off, nbr, int(d[0]), int(d[1]), d[2:2 + V], d[2 + V:2 + 2 * V], d[2 + 2 * V:])
+    if flags:
+        assert f"levels {int(d[0]) + 1} " in info  # the reference's loop: every level 0..kmax
+
+
 def test_prefetch_variant(emu, tmp_path):
     g = K.Graph.chung_lu(6000, 8)
     off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
