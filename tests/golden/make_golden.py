@@ -1,4 +1,5 @@
-"""Writes the committed golden fixtures under tests/golden/. Run here (not on the GPU box):
+"""Writes the committed golden fixtures under tests/golden/. Run where the reference sources and
+networkx are available (the fixtures themselves need neither):
 
     python tests/golden/make_golden.py
 
