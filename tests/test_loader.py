@@ -131,3 +131,21 @@ def test_pairs_rules():
     assert list(g.degrees) == [1, 1, 1, 0, 1]
     e = K.Graph.from_pairs([7], [7])
     assert e.V == 1 and e.arcs == 0
+
+
+def test_networkx_crosscheck_tool(tmp_path):
+    """oracle/nx_crosscheck.py (the reference notebook's check, offline) accepts the JSON core file
+    written for the oracle's core numbers and rejects one with a changed core."""
+    pytest.importorskip("networkx")
+    import oracle.nx_crosscheck as X
+    g = K.Graph.chung_lu(3000, 12)
+    txt = tmp_path / "g.txt"
+    g.write_edgelist(str(txt))
+    core, _ = O.bz(g.neighbors_offset, g.neighbors)
+    js = tmp_path / "k.json"
+    K.write_kcore_json(str(js), core)
+    assert X.crosscheck(str(txt), str(js)) == (True, 0)
+    bad = core.copy()
+    bad[int(np.flatnonzero(bad)[0])] += 1
+    K.write_kcore_json(str(js), bad)
+    assert X.crosscheck(str(txt), str(js)) == (False, 1)
