@@ -177,8 +177,9 @@ void kc_batch_info(const kc_batch* b, uint32_t* G, uint64_t* total_vertices, uin
  * its warp/block split in inc/common.h) ----
  * Times real peels of the resident graph (batch) and returns the fastest kc_params: for a graph,
  * coordinate descent over variant, block, heavy_threshold (the wave/block split), chunk_arcs and
- * lds_depth starting from the defaults; for a batch, block. Results are identical for every
- * candidate. cache_path (may be NULL) names a text file of tuned settings keyed by graph shape,
+ * lds_depth starting from the defaults; for a batch, block. Results must be identical for every
+ * candidate: a single-graph candidate is adopted only if its core numbers equal those of the
+ * default settings, and one that fails with a device-side status is skipped. cache_path (may be NULL) names a text file of tuned settings keyed by graph shape,
  * flags and device; a cached key returns at once with *trials = 0. */
 typedef struct {
     uint32_t reps;          /* timed peels per candidate (median); 0 = 2 */

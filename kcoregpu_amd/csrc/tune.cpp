@@ -5,7 +5,8 @@
 // threshold), chunk length, then the LDS frontier depth (sub-round variants) or the alive-list
 // compaction ratio and idle poll step (cascade-queue variant); each axis keeps its fastest value
 // (median of `reps` timed peels, device time) before the next axis is tried. Batch: workgroup size.
-// Every candidate produces identical core numbers, so tuning changes speed only.
+// Every candidate must produce identical core numbers; the tuner enforces it: a candidate is
+// adopted only if the core numbers of its last timed peel equal those of the default settings.
 //
 // Results can be kept in a text cache, one line per key:
 //   peel <V> <arcs> <dmax> <flags> <arch> <cus> : <variant> <block> <heavy> <chunk> <lds> <compact> <poll> <ms>
@@ -113,6 +114,11 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
     double cur_ms = 0;
     kc_status s = measure(cur, &cur_ms);
     if (s != KC_OK) return s;
+    std::vector<uint32_t> ref(V), got(V);
+    if ((s = kc_graph_download_core(c, g, ref.data())) != KC_OK) return s;
+    auto same = [&]() {
+        return kc_graph_download_core(c, g, got.data()) == KC_OK && got == ref;
+    };
     struct Axis {
         uint32_t kc_params::*field;
         std::vector<uint32_t> values;
@@ -129,7 +135,7 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
             if (fatal(s)) return s;
             continue;
         }
-        if (ms < cur_ms) cur = p, cur_ms = ms;
+        if (ms < cur_ms && same()) cur = p, cur_ms = ms;
     }
     std::vector<Axis> axes = {
         {&kc_params::block, {256, 512, 1024}},
@@ -152,7 +158,7 @@ kc_status kc_tune_peel(kc_ctx* c, kc_graph* g, const kc_tune_cfg* cfg, kc_params
                 if (fatal(s)) return s;
                 continue;
             }
-            if (ms < cur_ms) cur = p, cur_ms = ms;
+            if (ms < cur_ms && same()) cur = p, cur_ms = ms;
         }
     }
     *best = cur;
