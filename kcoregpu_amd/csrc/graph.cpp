@@ -97,7 +97,7 @@ int build_csr(const uint32_t* s, const uint32_t* t, uint64_t np, int nthreads, C
     const uint32_t* off = out.offsets.data();
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 4096)
     for (int64_t v = 0; v < (int64_t)V; ++v)
-        std::memcpy(nb + off[v], buf.data() + start[v], (size_t)deg[v] * 4);
+        if (deg[v]) std::memcpy(nb + off[v], buf.data() + start[v], (size_t)deg[v] * 4);
     return OK;
 }
 

@@ -51,3 +51,20 @@ def test_request_graphs_match_their_shape():
             assert g.V <= (2 * m + 11) // 12 + 1
         else:
             assert g.V <= 1 << max(6, int(np.ceil(np.log2(m / 16))))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_host_library_under_sanitizers(tmp_path):
+    src = os.path.join(os.path.dirname(HERE), "kcoregpu_amd", "csrc")
+    exe = tmp_path / "host_sanitize"
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fopenmp", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-o", str(exe),
+                    os.path.join(HERE, "cpp", "host_sanitize.cpp")] +
+                   [os.path.join(src, f) for f in ("graph.cpp", "generators.cpp", "graph_capi.cpp", "bincache.cpp")],
+                   check=True)
+    fixtures = sorted(os.path.join(HERE, "golden", "loader", f)
+                      for f in os.listdir(os.path.join(HERE, "golden", "loader")) if f.endswith(".txt"))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", TMPDIR=str(tmp_path))
+    r = subprocess.run([str(exe), *fixtures], capture_output=True, text=True, timeout=300, env=env, cwd=tmp_path)
+    assert r.returncode == 0 and "runtime error" not in r.stderr, r.stderr[-3000:]
+    assert r.stdout.startswith("ok")
