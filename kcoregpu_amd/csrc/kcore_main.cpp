@@ -95,7 +95,9 @@ int main(int argc, char* argv[]) {
         gp.reset(new Graph(ds));
     }
     Graph& g = *gp;
-    if (g.V == 0) return -1;  // the reference exits inside readFile on a missing file
+    // the reference exits inside readFile on a missing file (src/graph.cpp:63-66), before the
+    // Graph destructor could print "Deallocated... ": std::exit does not unwind this frame either
+    if (g.V == 0) std::exit(-1);
     std::cout << "******************  " << ds << " ****************** " << std::endl;
     std::cout << "V: " << g.V << std::endl;
     std::cout << "E: " << g.E << std::endl;

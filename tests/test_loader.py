@@ -149,3 +149,17 @@ def test_networkx_crosscheck_tool(tmp_path):
     bad[int(np.flatnonzero(bad)[0])] += 1
     K.write_kcore_json(str(js), bad)
     assert X.crosscheck(str(txt), str(js)) == (False, 1)
+
+
+def test_driver_error_lines_match_reference(tmp_path):
+    """`kcore` without an argument, and with a missing file, prints the reference's lines
+    (main.cu:45-48, src/graph.cpp:9-33,58-66) and exits with -1 before touching a GPU."""
+    exe = os.path.join(os.path.dirname(K._native.HIP_LIB), "..", "bin", "kcore")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 255 and r.stdout == "Please provide data file\n"
+    r = subprocess.run([exe, "missing.txt"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 255
+    assert r.stdout == ("Graph loading Started... \nserialized-missing.txt : File couldn't open\n"
+                        "Reading normal file... \nload graph file failed \n")
