@@ -204,12 +204,17 @@ kc_status kc_tune_batch(kc_ctx* ctx, kc_batch* b, const kc_tune_cfg* cfg, kc_par
  * the kernel writes the core numbers there itself; otherwise kc_serve_poll copies them down.
  * kc_serve_poll reports a request once its core numbers are in core_out and frees its region.
  * Requests of at least large_arcs arcs are peeled instead by the single-graph kernel over many
- * workgroups, on their own streams next to the resident kernel (charged to the same budget as
- * max(8 m + 16 n, the single-graph state: about 4.7 arcs + 28 n bytes + 2 MiB)).
+ * workgroups, on their own streams next to the resident kernel, with their whole single-graph
+ * state (about 4.7 arcs + 28 n bytes + 1.3 MiB) placed in their own arena region: they are
+ * charged max(8 m + 16 n, that state) and allocate no other device memory.
  * The CSR arrays and core_out must stay valid until the request's completion is returned.
  * Not thread-safe: one host thread drives a server.
  * The kernel never outlives the process: idle workgroups exit when the library's heartbeat
- * thread has been silent for lease_ms, and open servers are stopped at process exit. */
+ * thread has been silent for lease_ms, and open servers are stopped at process exit.
+ * While a serving kernel is resident, hipFree, hipHostFree and hipDeviceSynchronize called by
+ * the process wait for it (they synchronise the whole device) and so do not return before the
+ * server is closed; this library defers its own frees (kc_graph_free, kc_batch_free,
+ * kc_host_free) until no serving kernel is resident. */
 typedef struct kc_server kc_server;
 typedef struct {
     uint64_t hbm_budget_bytes; /* arena size; 0 = 8 GiB */
@@ -241,7 +246,10 @@ kc_status kc_serve_pending(const kc_server* s, uint32_t* queued, uint32_t* infli
 /* Stops the kernel after every admitted request has run; KC_ERR_DEVICE if it does not exit. */
 kc_status kc_serve_close(kc_server* s);
 double kc_serve_clock(void);
-/* Pinned host memory (fast asynchronous copies) for request buffers. */
+/* Pinned host memory (fast asynchronous copies; written directly by the serving kernel) for request
+ * buffers, in 4 KiB multiples. kc_host_free takes only kc_host_alloc blocks (else KC_ERR_ARG,
+ * also for a second free); a block freed while a serving kernel is resident is kept and handed
+ * out again by kc_host_alloc, and released once no serving kernel is resident. Thread-safe. */
 kc_status kc_host_alloc(uint64_t bytes, void** out);
 kc_status kc_host_free(void* p);
 

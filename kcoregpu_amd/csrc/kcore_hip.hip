@@ -114,7 +114,41 @@ struct kc_graph {
     uint32_t last_kmax = 0;
     uint32_t last_ntrace = 0;
     bool peeled = false;
+    bool external = false;        // arrays placed in caller memory (graph_upload_into): never freed
 };
+
+// chunks: only a vertex of degree d > heavy (>= kHeavyMin) is cut, into ceil(d / chunk)
+// <= d / chunk + 1 pieces, at most once per peel; there are fewer than arcs / kHeavyMin such vertices
+static uint32_t chunk_cap_of(uint64_t arcs) {
+    return (uint32_t)std::min<uint64_t>(arcs / kHeavyMin + arcs / kChunkMin + 64, 0xFFFFFFF0ull);
+}
+
+// The device arrays of a graph (V, padV, arcs, dmax and chunk_cap set), in placement order, with
+// their sizes in bytes; the alive lists only when `alive` (caller memory: always placed).
+static int graph_arrays(kc_graph* g, bool alive, void** ptr[13], size_t bytes[13]) {
+    int n = 0;
+    auto add = [&](void* p, size_t b) {
+        ptr[n] = static_cast<void**>(p);
+        bytes[n] = b;
+        ++n;
+    };
+    add(&g->d_off, ((size_t)g->V + 1) * 4);
+    add(&g->d_nbr, std::max<size_t>(g->arcs, 1) * 4);
+    add(&g->d_deg0, (size_t)g->padV * 4);
+    add(&g->d_core, (size_t)g->padV * 4);
+    add(&g->d_order, (size_t)g->padV * 4);
+    add(&g->d_work, (size_t)g->padV * 4);
+    add(&g->d_chunks, (size_t)g->chunk_cap * 8);
+    add(&g->d_shell, ((size_t)g->dmax + 3) * 4);
+    add(&g->d_wgmin, (size_t)kMaxGrid * 4);
+    add(&g->d_trace, (size_t)kTraceCap * 16);
+    add(&g->d_dbg, (size_t)kMaxGrid * 8 * 4);
+    add(&g->d_ctrl, (size_t)A_COUNT * kStride * 4);
+    if (alive) add(&g->d_alive, (size_t)g->padV * 8);
+    return n;
+}
+
+static uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 template <int BLOCK, bool PREFETCH, bool ASYNC>
 static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out) {
@@ -158,8 +192,12 @@ static kc_status peel_stepwise(kc_ctx* c, kc_graph* g, uint32_t flags, uint32_t*
                                uint32_t* launches_out, float* ms_out) {
     hipStream_t st = c->stream;
     uint32_t* ctr = g->d_ctrl;  // [0] order tail, [1] smallest degree above the level
-    uint32_t* h = nullptr;
-    KC_TRY(hipHostMalloc(&h, 64, hipHostMallocDefault));
+    struct Pinned {
+        uint32_t* p = nullptr;
+        ~Pinned() { lib_free_pinned(p); }
+    } pin;
+    if (const kc_status ps = kc_host_alloc(64, reinterpret_cast<void**>(&pin.p))) return ps;
+    uint32_t* h = pin.p;
     std::vector<uint32_t> shell(g->dmax + 3, 0);
     const uint32_t grid = std::min<uint32_t>(4096u, std::max<uint32_t>(1u, (g->V + 255) / 256));
     const uint32_t pgrid = (uint32_t)c->cus * 8;
@@ -209,15 +247,111 @@ static kc_status peel_stepwise(kc_ctx* c, kc_graph* g, uint32_t flags, uint32_t*
     }
     KC_TRY(hipEventRecord(c->ev1, st));
     KC_TRY(hipEventSynchronize(c->ev1));
-    (void)hipHostFree(h);
     if (rs != KC_OK) return rs;
     KC_TRY(hipGetLastError());
-    KC_TRY(hipMemcpy(g->d_shell, shell.data(), (size_t)(L + 2) * 4, hipMemcpyHostToDevice));
+    KC_TRY(hipMemcpyAsync(g->d_shell, shell.data(), (size_t)(L + 2) * 4, hipMemcpyHostToDevice, st));
+    KC_TRY(hipStreamSynchronize(st));
     KC_TRY(hipEventElapsedTime(ms_out, c->ev0, c->ev1));
     *kmax_out = L;
     *levels_out = levels;
     *launches_out = launches;
     return KC_OK;
+}
+
+// Builds a resident graph: its arrays are allocated one by one (mem == nullptr) or placed in
+// [mem, mem + bytes) at 256-byte boundaries (external); the CSR is copied in and checked on the
+// device before any peeling kernel may index with it.
+static kc_status upload_impl(kc_ctx* c, const kc_csr* h, uint8_t* mem, uint64_t bytes, kc_graph** out) {
+    if (!c || !h || !out || !h->offsets || (h->arcs && !h->neighbors)) return KC_ERR_ARG;
+    if (mem && (reinterpret_cast<uintptr_t>(mem) & 255u)) return KC_ERR_ARG;
+    // padV = V rounded up to 256 must keep the degree array below 4 GiB (one buffer descriptor)
+    if (h->V == 0 || h->V > (1u << 30) - 256 || h->arcs >= 0xFFFFFFFFull) return KC_ERR_CAPACITY;
+    if (h->offsets[0] != 0 || h->offsets[h->V] != h->arcs) return KC_ERR_ARG;
+    KC_TRY(hipSetDevice(c->device));
+    kc_graph* g = new (std::nothrow) kc_graph;
+    if (!g) return KC_ERR_OOM;
+    g->ctx = c;
+    g->V = h->V;
+    g->arcs = h->arcs;
+    g->padV = (h->V + 255u) & ~255u;
+    uint32_t dmax = 0;
+    for (uint32_t v = 0; v < h->V; ++v) {
+        if (h->offsets[v + 1] < h->offsets[v]) {
+            delete g;
+            return KC_ERR_ARG;
+        }
+        dmax = std::max(dmax, h->offsets[v + 1] - h->offsets[v]);
+    }
+    g->dmax = dmax;
+    g->chunk_cap = chunk_cap_of(h->arcs);
+    g->external = mem != nullptr;
+    auto fail = [&](hipError_t e) {
+        kc_graph_free(g);
+        return hip_status(e);
+    };
+    void** ptr[13];
+    size_t sz[13];
+    const int n = graph_arrays(g, g->external, ptr, sz);
+    hipError_t e;
+    if (g->external) {
+        uint64_t need = 0;
+        for (int i = 0; i < n; ++i) need += align256(sz[i]);
+        if (need > bytes) {
+            delete g;
+            return KC_ERR_CAPACITY;
+        }
+        uint64_t pos = 0;
+        for (int i = 0; i < n; ++i) {
+            *ptr[i] = mem + pos;
+            pos += align256(sz[i]);
+        }
+    } else {
+        for (int i = 0; i < n; ++i)
+            if ((e = hipMalloc(ptr[i], sz[i])) != hipSuccess) return fail(e);
+    }
+    if ((e = hipMemcpyAsync(g->d_off, h->offsets, ((size_t)h->V + 1) * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
+    if (h->arcs && (e = hipMemcpyAsync(g->d_nbr, h->neighbors, h->arcs * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
+    if (h->degrees) {
+        if ((e = hipMemcpyAsync(g->d_deg0, h->degrees, (size_t)h->V * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
+        if (g->padV > h->V) fill_kernel<uint32_t><<<1, 256, 0, c->stream>>>(g->d_deg0 + h->V, kSent, g->padV - h->V);
+    } else {
+        degrees_from_offsets<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_deg0, g->V, g->padV);
+    }
+    // chunk slots start empty (0): the ASYNC kernel waits on a claimed slot until it is written
+    fill_kernel<uint64_t><<<1024, 256, 0, c->stream>>>(g->d_chunks, 0ull, g->chunk_cap);
+    // reject a malformed CSR before any kernel indexes with it
+    fill_kernel<uint32_t><<<1, 64, 0, c->stream>>>(g->d_ctrl, 0u, 1);
+    validate_csr<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_nbr, g->V, g->arcs, g->d_ctrl);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    uint32_t bad = 0;
+    if ((e = hipMemcpyAsync(&bad, g->d_ctrl, 4, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e);
+    if (bad) {
+        kc_graph_free(g);
+        return KC_ERR_ARG;
+    }
+    *out = g;
+    return KC_OK;
+}
+
+uint64_t graph_bytes(uint32_t V, uint64_t arcs, uint32_t dmax) {
+    kc_graph g;
+    g.V = V;
+    g.padV = (V + 255u) & ~255u;
+    g.arcs = arcs;
+    g.dmax = dmax;
+    g.chunk_cap = chunk_cap_of(arcs);
+    void** ptr[13];
+    size_t sz[13];
+    const int n = graph_arrays(&g, true, ptr, sz);
+    uint64_t need = 0;
+    for (int i = 0; i < n; ++i) need += align256(sz[i]);
+    return need;
+}
+
+kc_status graph_upload_into(kc_ctx* c, const kc_csr* h, void* mem, uint64_t bytes, kc_graph** out) {
+    if (!mem) return KC_ERR_ARG;
+    return upload_impl(c, h, static_cast<uint8_t*>(mem), bytes, out);
 }
 
 extern "C" {
@@ -280,83 +414,17 @@ void kc_graph_free(kc_graph* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
-    for (void* p : {(void*)g->d_off, (void*)g->d_nbr, (void*)g->d_deg0, (void*)g->d_core,
-                    (void*)g->d_order, (void*)g->d_work, (void*)g->d_chunks, (void*)g->d_shell,
-                    (void*)g->d_wgmin, (void*)g->d_trace, (void*)g->d_dbg, (void*)g->d_ctrl,
-                    (void*)g->d_alive})
-        if (p) (void)hipFree(p);
+    if (!g->external) {
+        void** ptr[13];
+        size_t sz[13];
+        const int n = graph_arrays(g, true, ptr, sz);
+        for (int i = 0; i < n; ++i) lib_free_device(*ptr[i]);
+    }
     delete g;
 }
 
 kc_status kc_graph_upload(kc_ctx* c, const kc_csr* h, kc_graph** out) {
-    if (!c || !h || !out || !h->offsets || (h->arcs && !h->neighbors)) return KC_ERR_ARG;
-    // V < 2^30: the degree array must stay addressable by one buffer descriptor (< 4 GiB)
-    // padV = V rounded up to 256 must keep the degree array below 4 GiB (one buffer descriptor)
-    if (h->V == 0 || h->V > (1u << 30) - 256 || h->arcs >= 0xFFFFFFFFull) return KC_ERR_CAPACITY;
-    if (h->offsets[0] != 0 || h->offsets[h->V] != h->arcs) return KC_ERR_ARG;
-    KC_TRY(hipSetDevice(c->device));
-    kc_graph* g = new (std::nothrow) kc_graph;
-    if (!g) return KC_ERR_OOM;
-    g->ctx = c;
-    g->V = h->V;
-    g->arcs = h->arcs;
-    g->padV = (h->V + 255u) & ~255u;
-    uint32_t dmax = 0;
-    for (uint32_t v = 0; v < h->V; ++v) {
-        if (h->offsets[v + 1] < h->offsets[v]) {
-            delete g;
-            return KC_ERR_ARG;
-        }
-        dmax = std::max(dmax, h->offsets[v + 1] - h->offsets[v]);
-    }
-    g->dmax = dmax;
-    // chunks: only a vertex of degree d > heavy (>= kHeavyMin) is cut, into ceil(d / chunk)
-    // <= d / chunk + 1 pieces, at most once per peel; there are fewer than arcs / kHeavyMin
-    // such vertices
-    g->chunk_cap = (uint32_t)std::min<uint64_t>(h->arcs / kHeavyMin + h->arcs / kChunkMin + 64, 0xFFFFFFF0ull);
-    auto fail = [&](hipError_t e) {
-        kc_graph_free(g);
-        return hip_status(e);
-    };
-    hipError_t e;
-#define KC_ALLOC(ptr, bytes) \
-    if ((e = hipMalloc(&(ptr), (bytes))) != hipSuccess) return fail(e)
-    KC_ALLOC(g->d_off, ((size_t)h->V + 1) * 4);
-    KC_ALLOC(g->d_nbr, std::max<size_t>(h->arcs, 1) * 4);
-    KC_ALLOC(g->d_deg0, (size_t)g->padV * 4);
-    KC_ALLOC(g->d_core, (size_t)g->padV * 4);
-    KC_ALLOC(g->d_order, (size_t)g->padV * 4);
-    KC_ALLOC(g->d_work, (size_t)g->padV * 4);
-    KC_ALLOC(g->d_chunks, (size_t)g->chunk_cap * 8);
-    KC_ALLOC(g->d_shell, ((size_t)dmax + 3) * 4);
-    KC_ALLOC(g->d_wgmin, (size_t)kMaxGrid * 4);
-    KC_ALLOC(g->d_trace, (size_t)kTraceCap * 16);
-    KC_ALLOC(g->d_dbg, (size_t)kMaxGrid * 8 * 4);
-    KC_ALLOC(g->d_ctrl, (size_t)A_COUNT * kStride * 4);
-#undef KC_ALLOC
-    if ((e = hipMemcpyAsync(g->d_off, h->offsets, ((size_t)h->V + 1) * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
-    if (h->arcs && (e = hipMemcpyAsync(g->d_nbr, h->neighbors, h->arcs * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
-    if (h->degrees) {
-        if ((e = hipMemcpyAsync(g->d_deg0, h->degrees, (size_t)h->V * 4, hipMemcpyHostToDevice, c->stream)) != hipSuccess) return fail(e);
-        if (g->padV > h->V) fill_kernel<uint32_t><<<1, 256, 0, c->stream>>>(g->d_deg0 + h->V, kSent, g->padV - h->V);
-    } else {
-        degrees_from_offsets<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_deg0, g->V, g->padV);
-    }
-    // chunk slots start empty (0): the ASYNC kernel waits on a claimed slot until it is written
-    fill_kernel<uint64_t><<<1024, 256, 0, c->stream>>>(g->d_chunks, 0ull, g->chunk_cap);
-    // reject a malformed CSR before any kernel indexes with it
-    fill_kernel<uint32_t><<<1, 64, 0, c->stream>>>(g->d_ctrl, 0u, 1);
-    validate_csr<<<1024, 256, 0, c->stream>>>(g->d_off, g->d_nbr, g->V, g->arcs, g->d_ctrl);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-    uint32_t bad = 0;
-    if ((e = hipMemcpyAsync(&bad, g->d_ctrl, 4, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return fail(e);
-    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e);
-    if (bad) {
-        kc_graph_free(g);
-        return KC_ERR_ARG;
-    }
-    *out = g;
-    return KC_OK;
+    return upload_impl(c, h, nullptr, 0, out);
 }
 
 void kc_graph_info(const kc_graph* g, uint32_t* V, uint64_t* arcs, uint32_t* dmax) {
@@ -381,7 +449,10 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
         g->last_kmax = kmax;
         g->last_ntrace = 0;
         g->peeled = true;
-        if (core_out) KC_TRY(hipMemcpy(core_out, g->d_core, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+        if (core_out) {
+            KC_TRY(hipMemcpyAsync(core_out, g->d_core, (size_t)g->V * 4, hipMemcpyDeviceToHost, c->stream));
+            KC_TRY(hipStreamSynchronize(c->stream));
+        }
         if (kmax_out) *kmax_out = kmax;
         if (st) {
             memset(st, 0, sizeof(*st));
@@ -600,15 +671,18 @@ kc_status kc_selftest(kc_ctx* c) {
     for (int i = 0; i < 64; ++i) h_in[i] = (uint32_t)(i * 7 + 3) % 11 + (i == 40 ? 100000u : 0u);
     uint32_t *d_in = nullptr, *d_out = nullptr;
     KC_TRY(hipMalloc(&d_in, 256));
-    KC_TRY(hipMalloc(&d_out, 256));
-    hipError_t e = hipMemcpy(d_in, h_in, 256, hipMemcpyHostToDevice);
+    if (hipMalloc(&d_out, 256) != hipSuccess) {
+        lib_free_device(d_in);
+        return KC_ERR_OOM;
+    }
+    hipError_t e = hipMemcpyAsync(d_in, h_in, 256, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         selftest_scan_kernel<<<1, 64, 0, c->stream>>>(d_in, d_out);
-        e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(h_out, d_out, 256, hipMemcpyDeviceToHost, c->stream);
     }
-    if (e == hipSuccess) e = hipMemcpy(h_out, d_out, 256, hipMemcpyDeviceToHost);
-    (void)hipFree(d_in);
-    (void)hipFree(d_out);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    lib_free_device(d_in);
+    lib_free_device(d_out);
     KC_TRY(e);
     uint32_t acc = 0;
     for (int i = 0; i < 64; ++i) {
@@ -645,7 +719,7 @@ void kc_batch_free(kc_batch* b) {
     (void)hipStreamSynchronize(b->ctx->stream);
     for (void* p : {(void*)b->d_vtx, (void*)b->d_arc, (void*)b->d_off, (void*)b->d_nbr, (void*)b->d_deg0,
                     (void*)b->d_core, (void*)b->d_order, (void*)b->d_kmax, (void*)b->d_sched, (void*)b->d_ctrl})
-        if (p) (void)hipFree(p);
+        lib_free_device(p);
     for (auto& cg : b->graphs) {
         (void)hipGraphExecDestroy(cg.exec);
         (void)hipGraphDestroy(cg.graph);

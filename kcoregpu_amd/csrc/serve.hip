@@ -13,13 +13,20 @@
 // kc_serve_poll copies them down.
 //
 // Large requests (at least large_arcs arcs) would hold one workgroup for long: they go to a
-// large-request lane instead, where a worker thread with its own context and stream uploads the
-// request, peels it with the single-graph kernel over many workgroups (running beside the resident
-// kernel) and downloads the core numbers; large_lanes workers run at once.
+// large-request lane instead, where a worker thread with its own context and stream places the
+// request's whole single-graph state in its arena region, peels it with the single-graph kernel
+// over many workgroups (running beside the resident kernel) and downloads the core numbers;
+// large_lanes workers run at once. Lane contexts are opened before the resident kernel starts and
+// closed after it ends, and a lane allocates and frees no device memory.
 //
 // Lifetime safety: the kernel never outlives its process. A heartbeat thread advances a word in
 // the control page while the server is open; idle kernel workgroups exit when it stops advancing
 // for lease_ms, and an exit handler stops every open server before the HIP runtime shuts down.
+//
+// hipFree and hipHostFree synchronise the whole device, so while a serving kernel is resident
+// they would wait for it forever: the library's frees (lib_free_device, lib_free_pinned,
+// kc_host_free) go through one FreeRegistry (resident.h), which defers them until the last
+// resident kernel has ended and hands returned pinned blocks out again meanwhile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +44,7 @@
 
 #include "arena.h"
 #include "host_internal.h"
+#include "resident.h"
 #include "serve_kernel.h"
 
 using namespace kcdev;
@@ -104,12 +112,14 @@ struct kc_server {
     uint64_t next_index = 0;
     uint32_t inflight = 0;
     bool running = false;
+    bool resident = false;  // counted in the resident-kernel registry (launched, end not yet seen)
     std::thread hb;  // heartbeat
     std::mutex hb_m;
     std::condition_variable hb_cv;
     bool hb_quit = false;
     // large-request lane
     std::deque<Req> lqueue;         // waiting for a lane (arrival order), owned by the host thread
+    std::vector<kc_ctx*> lctx;      // one context per lane, open while the server is
     std::vector<std::thread> lanes;
     std::mutex lm;
     std::condition_variable lcv;
@@ -121,16 +131,38 @@ struct kc_server {
     std::vector<LaneDone> ldone;
     uint32_t lbusy = 0;             // admitted large requests not yet reported (host thread)
     bool lquit = false;
-    kc_status lerr = KC_OK;         // first worker failure (context could not be opened)
 };
 
 static void lanes_stop(kc_server* s);
 
 namespace {
 
-std::mutex g_reg_m;
+std::mutex g_reg_m;  // open servers
 std::vector<kc_server*> g_servers;
 bool g_atexit = false;
+
+struct HipFrees {
+    static int dev_free(void* p) { return (int)hipFree(p); }
+    static int host_free(void* p) { return (int)hipHostFree(p); }
+    static int host_alloc(void** p, uint64_t bytes) { return (int)hipHostMalloc(p, bytes, hipHostMallocDefault); }
+};
+FreeRegistry<HipFrees> g_frees;  // resident serving kernels and the frees deferred meanwhile
+
+kc_status free_status(int e) {
+    return e == FreeRegistry<HipFrees>::kBadBlock ? KC_ERR_ARG : hip_status((hipError_t)e);
+}
+
+void resident_begin(kc_server* s) {
+    g_frees.begin();
+    s->resident = true;
+}
+
+// the server's kernel has ended: once none is resident, the deferred blocks are freed
+void resident_end(kc_server* s) {
+    if (!s->resident) return;
+    s->resident = false;
+    g_frees.end();
+}
 
 void hb_stop(kc_server* s) {
     if (!s->hb.joinable()) return;
@@ -177,12 +209,11 @@ void reg_del(kc_server* s) {
 }  // namespace
 
 // A large-request lane: its own context (stream, events), one request at a time through the
-// single-graph path; results are reported through s->ldone.
-static void lane_main(kc_server* s) {
-    kc_ctx* c = nullptr;
-    const kc_status os = kc_open(s->ctx->device, &c);
+// single-graph path with the whole state inside the request's arena region; results are
+// reported through s->ldone.
+static void lane_main(kc_server* s, kc_ctx* c) {
+    (void)hipSetDevice(c->device);
     std::unique_lock<std::mutex> l(s->lm);
-    if (os != KC_OK && s->lerr == KC_OK) s->lerr = os;
     while (true) {
         s->lcv.wait(l, [&] { return s->lquit || !s->ljobs.empty(); });
         if (s->ljobs.empty()) break;  // quit, and nothing left to run
@@ -193,11 +224,11 @@ static void lane_main(kc_server* s) {
         cp.req_id = r.id;
         cp.t_submit = r.t_submit;
         cp.t_admit = r.t_admit;
-        kc_status st = c ? KC_OK : os;
-        if (c) {
+        kc_status st;
+        {
             kc_csr h{r.V, r.arcs, r.off, r.nbr, nullptr};
             kc_graph* g = nullptr;
-            st = kc_graph_upload(c, &h, &g);
+            st = graph_upload_into(c, &h, s->arena + r.base, r.charge, &g);
             if (st == KC_OK) {
                 kc_params p{};
                 p.flags = s->cfg.flags;
@@ -215,8 +246,6 @@ static void lane_main(kc_server* s) {
         l.lock();
         s->ldone.push_back({cp, r.base, r.charge});
     }
-    l.unlock();
-    if (c) kc_close(c);
 }
 
 static void lanes_stop(kc_server* s) {
@@ -230,11 +259,11 @@ static void lanes_stop(kc_server* s) {
     s->lanes.clear();
 }
 
-// charge of a large request: the single-graph state (offsets, neighbours, five vertex arrays, chunk
-// slots of 5/8 byte per arc, two alive lists, control and trace blocks)
-static uint64_t large_charge(uint32_t V, uint32_t arcs) {
-    const uint64_t state = 4ull * (V + 1) + 4ull * arcs + 28ull * V + (5ull * arcs) / 8 + (2ull << 20);
-    return std::max<uint64_t>(state, 4ull * arcs + 16ull * V);
+// charge of a large request: its whole single-graph state (offsets, neighbours, vertex arrays,
+// chunk slots, alive lists, shell offsets, control and trace blocks), which the lane places in
+// the request's arena region, and never less than the admission estimate 8 m + 16 n
+static uint64_t large_charge(uint32_t V, uint32_t arcs, uint32_t dmax) {
+    return std::max<uint64_t>(graph_bytes(V, arcs, dmax), 4ull * arcs + 16ull * V);
 }
 
 // Stops the serving kernel (the stop flag travels on the copy stream, after every publication,
@@ -258,15 +287,15 @@ static kc_status serve_release(kc_server* s) {
         }
         s->running = false;
     }
+    resident_end(s);
     hb_stop(s);
     reg_del(s);
+    for (kc_ctx* c : s->lctx) kc_close(c);
     if (s->cstream) (void)hipStreamSynchronize(s->cstream);
     for (Req& r : s->slot)
         if (r.ev) (void)hipEventDestroy(r.ev);
-    for (void* p : {(void*)s->arena, (void*)s->d_ring, (void*)s->d_head})
-        if (p) (void)hipFree(p);
-    for (void* p : {(void*)s->h_done, (void*)s->h_entry, (void*)s->h_ctl, (void*)s->h_one})
-        if (p) (void)hipHostFree(p);
+    for (void* p : {(void*)s->arena, (void*)s->d_ring, (void*)s->d_head}) lib_free_device(p);
+    for (void* p : {(void*)s->h_done, (void*)s->h_entry, (void*)s->h_ctl, (void*)s->h_one}) lib_free_pinned(p);
     if (s->kstream) (void)hipStreamDestroy(s->kstream);
     if (s->cstream) (void)hipStreamDestroy(s->cstream);
     delete s;
@@ -410,15 +439,30 @@ kc_status kc_serve_open(kc_ctx* c, const kc_serve_cfg* cfg, kc_server** out) {
     a.expired = s->d_head + kExpired;
     a.lease_ticks = (uint64_t)s->cfg.lease_ms * c->wall_rate_khz;
     a.slots = R;
+    if (s->cfg.large_arcs != 0xFFFFFFFFu)
+        for (uint32_t i = 0; i < s->cfg.large_lanes; ++i) {
+            kc_ctx* lc = nullptr;
+            const kc_status os = kc_open(c->device, &lc);
+            if (os != KC_OK) {
+                (void)serve_release(s);
+                return os;
+            }
+            s->lctx.push_back(lc);
+        }
+    KC_TRY(hipSetDevice(c->device));
+    s->running = true;  // from here on the release path stops the kernel
+    resident_begin(s);
     switch (s->cfg.block) {
         case 256: e = launch_serve<256>(a, s->cfg.workgroups, s->kstream); break;
         case 512: e = launch_serve<512>(a, s->cfg.workgroups, s->kstream); break;
         default: e = launch_serve<1024>(a, s->cfg.workgroups, s->kstream); break;
     }
-    if (e) return fail(e);
-    s->running = true;
-    if (s->cfg.large_arcs != 0xFFFFFFFFu)
-        for (uint32_t i = 0; i < s->cfg.large_lanes; ++i) s->lanes.emplace_back(lane_main, s);
+    if (e) {
+        s->running = false;
+        resident_end(s);
+        return fail(e);
+    }
+    for (kc_ctx* lc : s->lctx) s->lanes.emplace_back(lane_main, s, lc);
     *out = s;
     return KC_OK;
 }
@@ -438,8 +482,16 @@ kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32
     q.out_pinned = core_out && is_pinned(core_out);
     q.large = !s->lanes.empty() && q.arcs >= s->cfg.large_arcs;
     q.r = serve_region(q.V, q.arcs);
-    q.charge = q.large ? large_charge(q.V, q.arcs)
-                       : std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
+    if (q.large) {
+        uint32_t dmax = 0;
+        for (uint32_t v = 0; v < q.V; ++v) {
+            if (g->offsets[v + 1] < g->offsets[v]) return KC_ERR_ARG;
+            dmax = std::max(dmax, g->offsets[v + 1] - g->offsets[v]);
+        }
+        q.charge = large_charge(q.V, q.arcs, dmax);
+    } else {
+        q.charge = std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
+    }
     if (q.charge > s->cfg.hbm_budget_bytes) return KC_ERR_CAPACITY;
     q.t_submit = now_s();
     (q.large ? s->lqueue : s->queue).push_back(q);
@@ -498,6 +550,7 @@ kc_status kc_serve_poll(kc_server* s, kc_completion* out, uint32_t cap, uint32_t
     // a kernel that ended while the server is open (its lease ran out) is reported, not waited on
     if (n == 0 && s->inflight && hipStreamQuery(s->kstream) == hipSuccess) {
         s->running = false;
+        resident_end(s);
         return KC_ERR_DEVICE;
     }
     return serve_admit(s);
@@ -519,15 +572,13 @@ kc_status kc_serve_close(kc_server* s) {
 
 kc_status kc_host_alloc(uint64_t bytes, void** out) {
     if (!out) return KC_ERR_ARG;
-    *out = nullptr;
-    if (!bytes) return KC_OK;
-    KC_TRY(hipHostMalloc(out, bytes, hipHostMallocDefault));
-    return KC_OK;
+    return free_status(g_frees.alloc_pinned(bytes, out));
 }
 
-kc_status kc_host_free(void* p) {
-    if (p) KC_TRY(hipHostFree(p));
-    return KC_OK;
-}
+kc_status kc_host_free(void* p) { return free_status(g_frees.free_pinned(p)); }
 
 }  // extern "C"
+
+void lib_free_device(void* p) { (void)g_frees.free_device(p); }
+
+void lib_free_pinned(void* p) { (void)g_frees.free_host(p); }

@@ -206,3 +206,40 @@ def test_serve_large_request_lane(engine):
         assert got[i]["status"] == 0, i
         assert (got[i]["core"] == ex[i][0]).all(), i
         assert got[i]["kmax"] == ex[i][1], i
+
+
+def test_serve_frees_while_resident(engine):
+    """hipFree and hipHostFree synchronise the whole device, so with the serving kernel resident
+    they would wait for it forever. Pinned result buffers dropped mid-stream, and a graph uploaded,
+    peeled and freed beside the server, must neither hang nor disturb it: the library defers its
+    frees until no serving kernel is resident and hands returned pinned blocks out again."""
+    import ctypes as C
+    import gc
+
+    from kcoregpu_amd import _native as N
+    gs = requests(20, 40)
+    ex = expected(gs)
+    say("serve: frees with the kernel resident")
+    with engine.serve() as srv:
+        for i, g in enumerate(gs[:20]):
+            srv.submit(g, req_id=i)
+        for c in srv.drain(timeout_s=60):
+            assert (c["core"] == ex[c["req_id"]][0]).all(), c["req_id"]
+        gc.collect()  # the pinned result buffers are released here, with the kernel resident
+        # a returned block is handed out again (64 MiB: no result buffer is in its size range);
+        # a second free of it is refused
+        p = C.c_void_p()
+        assert N.hip().kc_host_alloc(64 << 20, C.byref(p)) == 0 and p.value
+        assert N.hip().kc_host_free(p) == 0
+        q = C.c_void_p()
+        assert N.hip().kc_host_alloc((64 << 20) - 100, C.byref(q)) == 0 and q.value == p.value
+        assert N.hip().kc_host_free(q) == 0
+        assert N.hip().kc_host_free(q) != 0
+        dg = engine.upload(gs[0])
+        r = dg.peel()
+        assert (r.core == ex[0][0]).all()
+        dg.free()
+        for i, g in enumerate(gs[20:], start=20):
+            srv.submit(g, req_id=i)
+        for c in srv.drain(timeout_s=60):
+            assert (c["core"] == ex[c["req_id"]][0]).all(), c["req_id"]

@@ -1,6 +1,7 @@
 """Host-side logic that runs without a GPU: the serving arena's first-fit allocator (compiled from
-its header with g++ and checked against a brute-force model) and the served-workload request
-generator's distribution."""
+its header with g++ and checked against a brute-force model), the registry that defers frees while
+a serving kernel is resident, the served-workload request generator's distribution, and the host
+library under sanitizers."""
 import os
 import shutil
 import subprocess
@@ -21,6 +22,20 @@ def test_arena_first_fit_against_model(tmp_path):
     r = subprocess.run([str(exe), "30"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok")
+
+
+@pytest.mark.parametrize("sanitizer", ["address,undefined", "thread"])
+def test_deferred_free_registry(tmp_path, sanitizer):
+    """hipFree / hipHostFree synchronise the whole device and would wait for a resident serving
+    kernel: the registry that defers the library's frees meanwhile (resident.h), with a fake
+    backend, single-threaded and under eight threads, with ASan+UBSan and with TSan."""
+    exe = tmp_path / "resident_test"
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-pthread", f"-fsanitize={sanitizer}", "-o", str(exe),
+                    os.path.join(HERE, "cpp", "resident_test.cpp")], check=True)
+    r = subprocess.run([str(exe), "20000" if sanitizer != "thread" else "5000"], capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1"))
+    assert r.returncode == 0 and "WARNING: ThreadSanitizer" not in r.stderr, r.stdout + r.stderr[-3000:]
+    assert "ok host_allocs" in r.stdout
 
 
 def test_request_shape_distribution():
