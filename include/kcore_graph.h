@@ -75,6 +75,9 @@ void kcg_graph_arrays(const kcg_graph* g, const uint32_t** offsets, const uint32
 int kcg_write_kcore_json(const char* path, uint32_t V, const uint32_t* core);
 /* Raw little-endian uint32[V] core dump. */
 int kcg_write_core_u32(const char* path, uint32_t V, const uint32_t* core);
+/* Text core dump, one number per line, each followed by '\n': the form of the degree block of the
+ * serialized cache (Graph::writeSerialized, src/graph.cpp:35-54). No reference counterpart. */
+int kcg_write_core_text(const char* path, uint32_t V, const uint32_t* core);
 
 /* ---- seeded synthetic generators (deterministic for a given seed, any thread count) ---- */
 

@@ -270,6 +270,22 @@ def write_kcore_json(path: str, core: np.ndarray) -> None:
         raise KCoreError(st, "writeKCoreToDisk")
 
 
+def write_core_u32(path: str, core: np.ndarray) -> None:
+    """Raw little-endian uint32[V] core dump."""
+    c = np.ascontiguousarray(core, dtype=np.uint32)
+    st = N.host().kcg_write_core_u32(os.fsencode(path), c.size, _u32(c))
+    if st:
+        raise KCoreError(st, "write_core_u32")
+
+
+def write_core_text(path: str, core: np.ndarray) -> None:
+    """One core number per line (the form of the serialized cache's degree block)."""
+    c = np.ascontiguousarray(core, dtype=np.uint32)
+    st = N.host().kcg_write_core_text(os.fsencode(path), c.size, _u32(c))
+    if st:
+        raise KCoreError(st, "write_core_text")
+
+
 def splitmix64(x: int) -> int:
     return N.host().kcg_splitmix64(x & 0xFFFFFFFFFFFFFFFF)
 

@@ -99,6 +99,7 @@ def host() -> C.CDLL:
         L.kcg_graph_arrays.restype = None
         L.kcg_write_kcore_json.argtypes = [C.c_char_p, C.c_uint32, u32p]
         L.kcg_write_core_u32.argtypes = [C.c_char_p, C.c_uint32, u32p]
+        L.kcg_write_core_text.argtypes = [C.c_char_p, C.c_uint32, u32p]
         L.kcg_gen_er.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, pp]
         L.kcg_gen_chung_lu.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_uint64, C.c_int, pp]

@@ -251,8 +251,9 @@ int write_serialized(const Csr& g, const std::string& path) {
     if (!f) return ERR_IO;
     std::string buf;
     buf.reserve(1 << 22);
+    bool short_write = false;
     auto flush = [&]() {
-        std::fwrite(buf.data(), 1, buf.size(), f);
+        short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
         buf.clear();
     };
     put_u32(buf, g.V, '\n');
@@ -270,7 +271,26 @@ int write_serialized(const Csr& g, const std::string& path) {
         if (buf.size() > (1 << 22) - 64) flush();
     }
     flush();
-    return std::fclose(f) == 0 ? OK : ERR_IO;
+    return (std::fclose(f) == 0 && !short_write) ? OK : ERR_IO;
+}
+
+// One core number per line, each followed by '\n': the form of the degree block of the
+// serialized cache (src/graph.cpp:35-54), so the same tools read both.
+int write_core_text(const std::string& path, uint32_t V, const uint32_t* core) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return ERR_IO;
+    std::string buf;
+    buf.reserve(1 << 22);
+    bool short_write = false;
+    for (uint32_t i = 0; i < V; ++i) {
+        put_u32(buf, core[i], '\n');
+        if (buf.size() > (1 << 22) - 64) {
+            short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
+            buf.clear();
+        }
+    }
+    short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
+    return (std::fclose(f) == 0 && !short_write) ? OK : ERR_IO;
 }
 
 namespace {

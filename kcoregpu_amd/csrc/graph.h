@@ -44,6 +44,7 @@ int read_serialized(const std::string& path, Csr& out);
 int write_serialized(const Csr& g, const std::string& path);
 int write_kcore_json(const std::string& path, uint32_t V, const uint32_t* core);
 int write_core_u32(const std::string& path, uint32_t V, const uint32_t* core);
+int write_core_text(const std::string& path, uint32_t V, const uint32_t* core);
 int write_edgelist(const Csr& g, const std::string& path);
 
 std::string basename_of(const std::string& path);

@@ -111,6 +111,11 @@ int kcg_write_core_u32(const char* path, uint32_t V, const uint32_t* core) {
     return kcg::write_core_u32(path, V, core);
 }
 
+int kcg_write_core_text(const char* path, uint32_t V, const uint32_t* core) {
+    if (!path || (!core && V)) return KCG_ERR_ARG;
+    return kcg::write_core_text(path, V, core);
+}
+
 int kcg_gen_er(uint32_t n, uint64_t m, uint64_t seed, int nthreads, kcg_graph** out) {
     if (!out) return KCG_ERR_ARG;
     kcg::Csr c;

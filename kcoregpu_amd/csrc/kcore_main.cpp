@@ -9,6 +9,8 @@
 //
 // Extra options (after the edge list), none needed for the reference behaviour:
 //   --core-out PATH   write the core numbers as raw little-endian uint32[V]
+//   --core-text PATH  write the core numbers as text, one per line (the serialized cache's
+//                     degree-block form)
 //   --json            write ./output/pkc-kcore-<edgelist> like Graph::writeKCoreToDisk
 //   --order-out PATH  write the peel order (uint32[V]) and PATH.shells (k-shell starts)
 //   --variant NAME    run only one variant: wave | lds | prefetch | async | auto | stepwise (the
@@ -57,7 +59,7 @@ int main(int argc, char* argv[]) {
         return -1;
     }
     std::string ds = argv[1];
-    std::string core_out, order_out, only;
+    std::string core_out, core_text, order_out, only;
     bool json = false, stats = false, binary = false;
     int reps = 1;
     for (int i = 2; i < argc; ++i) {
@@ -70,6 +72,7 @@ int main(int argc, char* argv[]) {
             return argv[++i];
         };
         if (a == "--core-out") core_out = next();
+        else if (a == "--core-text") core_text = next();
         else if (a == "--order-out") order_out = next();
         else if (a == "--json") json = true;
         else if (a == "--stats") stats = true;
@@ -151,7 +154,7 @@ int main(int argc, char* argv[]) {
         }
     }
     g.kmax = kmax;
-    if (!rc && (!core_out.empty() || json)) {
+    if (!rc && (!core_out.empty() || !core_text.empty() || json)) {
         std::vector<uint32_t> core(g.V);
         s = kc_graph_download_core(ctx, dg, core.data());
         if (s != KC_OK) {
@@ -160,6 +163,10 @@ int main(int argc, char* argv[]) {
         } else {
             if (!core_out.empty() && kcg::write_core_u32(core_out, g.V, core.data()) != kcg::OK) {
                 std::cout << core_out << " : File couldn't open" << std::endl;
+                rc = -1;
+            }
+            if (!core_text.empty() && kcg::write_core_text(core_text, g.V, core.data()) != kcg::OK) {
+                std::cout << core_text << " : File couldn't open" << std::endl;
                 rc = -1;
             }
             if (json) kcg::write_kcore_json(Graph::output_dir + "pkc-kcore-" + ds, g.V, core.data());

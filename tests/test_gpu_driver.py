@@ -30,7 +30,8 @@ def test_driver_lines_cache_and_outputs(tmp_path):
     os.makedirs(tmp_path / "output")
     core, kmax = O.bz(g.neighbors_offset, g.neighbors)
 
-    out = run(tmp_path, "er.txt", "--core-out", "core.u32", "--json", "--order-out", "order.u32")
+    out = run(tmp_path, "er.txt", "--core-out", "core.u32", "--core-text", "core.txt", "--json", "--order-out",
+              "order.u32")
     assert out[0] == "Graph loading Started... "
     assert out[1] == "serialized-er.txt : File couldn't open"
     assert out[2] == "Reading normal file... "
@@ -48,6 +49,7 @@ def test_driver_lines_cache_and_outputs(tmp_path):
 
     got = np.fromfile(tmp_path / "core.u32", np.uint32)
     assert (got == core).all()
+    assert (tmp_path / "core.txt").read_text() == "".join(f"{c}\n" for c in core)
     order = np.fromfile(tmp_path / "order.u32", np.uint32)
     assert O.check_certificate(g.neighbors_offset, g.neighbors, got, order) == 0
     shells = np.fromfile(tmp_path / "order.u32.shells", np.uint32)

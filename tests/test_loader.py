@@ -36,6 +36,18 @@ def test_loader_matches_reference_fixture(case, tmp_path):
     j = tmp_path / "core.json"
     K.write_kcore_json(str(j), z["core"])
     assert j.read_bytes() == open(os.path.join(exp, "pkc-kcore.json"), "rb").read()
+    # the text core dump has the form of the degree block the reference's writer produced:
+    # written from the degrees, it is exactly that block of the reference's serialized bytes
+    t = tmp_path / "deg.txt"
+    K.write_core_text(str(t), z["deg"])
+    ref = open(os.path.join(exp, "serialized"), "rb").read()
+    head = f"{int(z['V'])}\n{int(z['E'])}\n".encode()
+    assert ref.startswith(head + t.read_bytes()) and ref[len(head) + t.stat().st_size:].startswith(b"0 ")
+    K.write_core_text(str(t), z["core"])
+    assert [int(x) for x in t.read_text().split("\n")[:-1]] == list(z["core"])
+    u = tmp_path / "core.u32"
+    K.write_core_u32(str(u), z["core"])
+    assert (np.fromfile(u, np.uint32) == z["core"]).all()
 
 
 def test_load_cached_uses_basename_cache(tmp_path):
@@ -63,6 +75,17 @@ def test_json_writer_no_nonzero_core(tmp_path):
     j = tmp_path / "c.json"
     K.write_kcore_json(str(j), np.zeros(4, np.uint32))
     assert j.read_bytes() == b"{ "
+
+
+def test_core_dumps_of_empty_and_large_values(tmp_path):
+    t = tmp_path / "c.txt"
+    K.write_core_text(str(t), np.zeros(0, np.uint32))
+    assert t.read_bytes() == b""
+    big = np.array([0, 4294967295, 7, 1000000], np.uint32)
+    K.write_core_text(str(t), big)
+    assert t.read_bytes() == b"0\n4294967295\n7\n1000000\n"
+    with pytest.raises(K.KCoreError):
+        K.write_core_text(str(tmp_path / "no" / "such" / "dir.txt"), big)
 
 
 @pytest.mark.skipif(not os.path.exists(O.REF_LOADER), reason="reference loader not built here")
