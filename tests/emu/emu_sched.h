@@ -8,6 +8,9 @@
 //    reported as an error, since this project's kernels never rely on partial-wave semantics);
 //  * a workgroup barrier completes once every work-item that has not exited waits at it;
 //  * a sleep just yields. When a whole pass finds nothing but sleepers, the OS thread yields.
+// In weak-memory mode (KCEMU_WEAK=1) each work-item has a store buffer (see hip/hip_runtime.h):
+// at every yield each buffered store completes with probability 1/3 (in random order); a sleep,
+// the end of the work-item and a pass in which nothing could run complete them all.
 // Deadlocks inside a workgroup (every live fiber blocked on something that cannot complete)
 // abort with a message; waits on other workgroups are the kernels' own, bounded by their wall
 // clock limits (clock_ticks() counts 10 ns ticks).
@@ -28,12 +31,20 @@ namespace kcemu {
 
 enum State { RUN = 0, WAIT_WAVE, WAIT_BAR, DONE };
 
+struct SbEntry {
+    void* p;
+    uint64_t v;
+    int size;
+};
+
 struct Fiber {
     ucontext_t ctx;
     std::vector<char> stack;
     int state = RUN;
     int op = 0;
     uint64_t a = 0, b = 0, r = 0;
+    std::vector<SbEntry> sb;  // weak-memory store buffer, program order
+    uint64_t rng = 0;
 };
 
 struct Group {
@@ -46,6 +57,88 @@ struct Group {
 };
 
 inline thread_local Group* g_grp = nullptr;
+inline const bool g_weak = [] {
+    const char* e = std::getenv("KCEMU_WEAK");
+    return e && *e && *e != '0';
+}();
+
+inline void sb_complete(const SbEntry& e) {
+    switch (e.size) {
+        case 1: __atomic_store_n(static_cast<uint8_t*>(e.p), (uint8_t)e.v, __ATOMIC_SEQ_CST); break;
+        case 2: __atomic_store_n(static_cast<uint16_t*>(e.p), (uint16_t)e.v, __ATOMIC_SEQ_CST); break;
+        case 4: __atomic_store_n(static_cast<uint32_t*>(e.p), (uint32_t)e.v, __ATOMIC_SEQ_CST); break;
+        default: __atomic_store_n(static_cast<uint64_t*>(e.p), e.v, __ATOMIC_SEQ_CST); break;
+    }
+}
+inline void sb_drain_fiber(Fiber& f) {
+    for (const SbEntry& e : f.sb) sb_complete(e);
+    f.sb.clear();
+}
+inline uint64_t sb_rand(Fiber& f) {
+    f.rng ^= f.rng << 13;
+    f.rng ^= f.rng >> 7;
+    f.rng ^= f.rng << 17;
+    return f.rng;
+}
+// some buffered stores complete: each with probability 1/3, the rest keep their order
+inline void sb_some(Fiber& f) {
+    size_t k = 0;
+    for (size_t i = 0; i < f.sb.size(); ++i) {
+        if (sb_rand(f) % 3 == 0)
+            sb_complete(f.sb[i]);
+        else
+            f.sb[k++] = f.sb[i];
+    }
+    f.sb.resize(k);
+}
+inline Fiber* sb_self() { return (g_weak && g_grp && g_grp->cur >= 0) ? &g_grp->f[g_grp->cur] : nullptr; }
+inline bool overlaps(const SbEntry& e, const void* p, int size) {
+    const char* a = static_cast<const char*>(e.p);
+    const char* b = static_cast<const char*>(p);
+    return a < b + size && b < a + e.size;
+}
+
+inline bool sb_put(void* p, uint64_t v, int size) {
+    Fiber* f = sb_self();
+    if (!f) return false;
+    for (SbEntry& e : f->sb)
+        if (overlaps(e, p, size)) {
+            if (e.p == p && e.size == size) {  // a later store to the same word replaces it
+                e.v = v;
+                return true;
+            }
+            sb_drain_fiber(*f);
+            break;
+        }
+    f->sb.push_back(SbEntry{p, v, size});
+    return true;
+}
+inline bool sb_get(const void* p, int size, uint64_t* v) {
+    Fiber* f = sb_self();
+    if (!f) return false;
+    for (const SbEntry& e : f->sb)
+        if (overlaps(e, p, size)) {
+            if (e.p == p && e.size == size) {
+                *v = e.v;
+                return true;
+            }
+            sb_drain_fiber(*f);
+            return false;
+        }
+    return false;
+}
+inline void sb_touch(const void* p, int size) {
+    Fiber* f = sb_self();
+    if (!f) return;
+    for (const SbEntry& e : f->sb)
+        if (overlaps(e, p, size)) {
+            sb_drain_fiber(*f);
+            return;
+        }
+}
+inline void sb_drain() {
+    if (Fiber* f = sb_self()) sb_drain_fiber(*f);
+}
 
 inline kcemu_dim3 tid3() { return kcemu_dim3{(uint32_t)g_grp->cur, 0, 0}; }
 inline kcemu_dim3 bid3() { return kcemu_dim3{g_grp->bid, 0, 0}; }
@@ -55,6 +148,7 @@ inline uint32_t lane() { return (uint32_t)g_grp->cur & 63u; }
 
 inline void yield_to_sched() {
     Fiber& me = g_grp->f[g_grp->cur];
+    if (g_weak) sb_some(me);
     swapcontext(&me.ctx, &g_grp->sched);
 }
 
@@ -83,7 +177,10 @@ inline int sync_block_or(int x) {
     return (int)me.r;
 }
 
-inline void nap() { yield_to_sched(); }
+inline void nap() {
+    sb_drain();  // a spinning work-item's stores complete eventually
+    yield_to_sched();
+}
 
 inline uint64_t clock_ticks() {
     using namespace std::chrono;
@@ -160,6 +257,7 @@ inline void resolve_wave(Group& g, int w) {
 inline void trampoline(uint32_t hi, uint32_t lo) {
     Group* g = reinterpret_cast<Group*>(((uintptr_t)hi << 32) | lo);
     g->body();
+    sb_drain_fiber(g->f[g->cur]);  // a kernel's stores are complete when it ends
     g->f[g->cur].state = DONE;
     swapcontext(&g->f[g->cur].ctx, &g->sched);
 }
@@ -176,6 +274,7 @@ inline void run_group(Group& g) {
         f.ctx.uc_link = nullptr;
         const uintptr_t p = reinterpret_cast<uintptr_t>(&g);
         makecontext(&f.ctx, (void (*)())trampoline, 2, (uint32_t)(p >> 32), (uint32_t)p);
+        f.rng = 0x9E3779B97F4A7C15ull ^ ((uint64_t)g.bid << 32) ^ (uint64_t)(t + 1) * 0xD1B54A32D192ED03ull;
     }
     const int nw = (int)((g.block + 63) / 64);
     while (true) {
@@ -236,7 +335,11 @@ inline void run_group(Group& g) {
                 if (g.f[t].state == WAIT_BAR) fail("workgroup barrier not reached by every live work-item", g.bid, (int)t);
             fail("wave operation never completes", g.bid, 0);
         }
-        if (!progressed) std::this_thread::yield();
+        if (!progressed) {
+            // nothing could run: every buffered store completes (the GPU's do, eventually)
+            for (uint32_t t = 0; t < g.block; ++t) sb_drain_fiber(g.f[t]);
+            std::this_thread::yield();
+        }
         (void)any_live;
     }
     g_grp = nullptr;

@@ -4,6 +4,13 @@
 // workgroup barriers are synchronisation points at which the emulator (emu_sched.h) lines the
 // lanes up. LDS (__shared__) is thread_local, i.e. per workgroup. Memory is host memory; atomics
 // map to the compiler's __atomic builtins. This is test infrastructure, not a HIP runtime.
+//
+// Weak-memory mode (environment KCEMU_WEAK=1): a work-item's relaxed atomic stores wait in its own
+// store buffer, invisible to every other work-item, until it drains them (s_waitcnt, a fence, a
+// release store, an atomic read-modify-write of a buffered address) or the scheduler lets some of
+// them complete at a random later point; its own loads see them. Stores to different addresses
+// thus become visible in any order and late, as in-flight stores on the GPU may, and a kernel that
+// publishes data without draining first is caught. Workgroup barriers drain nothing.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -47,6 +54,11 @@ void sync_block();
 int sync_block_or(int x);
 void nap();
 uint64_t clock_ticks();
+// weak-memory store buffer of the running work-item (emu_sched.h)
+bool sb_put(void* p, uint64_t v, int size);        // false: store directly instead
+bool sb_get(const void* p, int size, uint64_t* v);  // forwarded from the own buffer
+void sb_touch(const void* p, int size);             // drain if an entry overlaps [p, p+size)
+void sb_drain();
 enum Op { OP_BALLOT = 1, OP_FIRSTLANE, OP_READLANE, OP_DPP, OP_SHFL_XOR, OP_WAVE_BARRIER };
 }  // namespace kcemu
 
@@ -82,8 +94,8 @@ inline int __builtin_amdgcn_update_dpp(int old, int v, int ctrl, int row_mask, i
 }
 inline void __builtin_amdgcn_wave_barrier() { (void)kcemu::coll(kcemu::OP_WAVE_BARRIER, 0, 0); }
 #define __builtin_amdgcn_s_sleep(n) kcemu::nap()
-#define __builtin_amdgcn_s_waitcnt(n) __atomic_thread_fence(__ATOMIC_SEQ_CST)
-#define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+#define __builtin_amdgcn_s_waitcnt(n) (kcemu::sb_drain(), __atomic_thread_fence(__ATOMIC_SEQ_CST))
+#define __builtin_amdgcn_fence(order, scope) (kcemu::sb_drain(), __atomic_thread_fence(__ATOMIC_SEQ_CST))
 
 // buffer resources: base + size; loads past the size return zeros, as on the GPU
 struct kcemu_rsrc {
@@ -104,6 +116,7 @@ inline kcemu_u32x4 __builtin_amdgcn_raw_buffer_load_b128(kcemu_rsrc r, int off, 
     const uint64_t o = (uint32_t)off;
     if (o + 16 <= r.bytes) {
         const uint32_t* p = reinterpret_cast<const uint32_t*>(r.base + o);
+        kcemu::sb_touch(p, 16);
         v.x = __atomic_load_n(p, __ATOMIC_RELAXED);
         v.y = __atomic_load_n(p + 1, __ATOMIC_RELAXED);
         v.z = __atomic_load_n(p + 2, __ATOMIC_RELAXED);
@@ -112,15 +125,34 @@ inline kcemu_u32x4 __builtin_amdgcn_raw_buffer_load_b128(kcemu_rsrc r, int off, 
     return v;
 }
 
-#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), __ATOMIC_SEQ_CST)
-#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
-#define __hip_atomic_fetch_add(p, v, order, scope) __atomic_fetch_add((p), (v), __ATOMIC_SEQ_CST)
-#define __hip_atomic_fetch_sub(p, v, order, scope) __atomic_fetch_sub((p), (v), __ATOMIC_SEQ_CST)
-#define __hip_atomic_fetch_or(p, v, order, scope) __atomic_fetch_or((p), (v), __ATOMIC_SEQ_CST)
-#define __hip_atomic_fetch_min(p, v, order, scope) kcemu_fetch_min((p), (v))
-#define __hip_atomic_fetch_max(p, v, order, scope) kcemu_fetch_max((p), (v))
+template <class T>
+inline T kcemu_load(T* p) {
+    uint64_t v;
+    if (kcemu::sb_get(p, (int)sizeof(T), &v)) return (T)v;
+    return __atomic_load_n(p, __ATOMIC_SEQ_CST);
+}
+template <class T, class U>
+inline void kcemu_store(T* p, U v, int order) {
+    const T x = (T)v;
+    if (order == __ATOMIC_RELAXED && kcemu::sb_put(p, (uint64_t)x, (int)sizeof(T))) return;
+    kcemu::sb_drain();  // a release store orders the work-item's earlier stores before it
+    __atomic_store_n(p, x, __ATOMIC_SEQ_CST);
+}
+template <class T>
+inline T* kcemu_rmw(T* p) {
+    kcemu::sb_touch(p, (int)sizeof(T));  // an own buffered store to the address goes first
+    return p;
+}
+
+#define __hip_atomic_load(p, order, scope) kcemu_load((p))
+#define __hip_atomic_store(p, v, order, scope) kcemu_store((p), (v), (order))
+#define __hip_atomic_fetch_add(p, v, order, scope) __atomic_fetch_add(kcemu_rmw(p), (v), __ATOMIC_SEQ_CST)
+#define __hip_atomic_fetch_sub(p, v, order, scope) __atomic_fetch_sub(kcemu_rmw(p), (v), __ATOMIC_SEQ_CST)
+#define __hip_atomic_fetch_or(p, v, order, scope) __atomic_fetch_or(kcemu_rmw(p), (v), __ATOMIC_SEQ_CST)
+#define __hip_atomic_fetch_min(p, v, order, scope) kcemu_fetch_min(kcemu_rmw(p), (v))
+#define __hip_atomic_fetch_max(p, v, order, scope) kcemu_fetch_max(kcemu_rmw(p), (v))
 #define __hip_atomic_compare_exchange_strong(p, e, d, o1, o2, scope) \
-    __atomic_compare_exchange_n((p), (e), (d), false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)
+    __atomic_compare_exchange_n(kcemu_rmw(p), (e), (d), false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)
 
 template <class T>
 inline T kcemu_fetch_min(T* p, T v) {

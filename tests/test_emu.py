@@ -1,8 +1,12 @@
 """The device kernels run on the CPU through the wave64 emulator (tests/emu): the same headers the
 GPU build compiles, with every wave operation, workgroup barrier and atomic emulated, several
 workgroups running at once. Results must equal the oracle's. This checks the kernels' logic —
-indexing, queue protocols, termination, level skipping, compaction, validation — without a GPU;
-it cannot check the GPU memory model (the host is more strongly ordered)."""
+indexing, queue protocols, termination, level skipping, compaction, validation — without a GPU.
+The weak-memory runs (KCEMU_WEAK=1) also hold every work-item's relaxed stores in a private store
+buffer until it drains them (s_waitcnt, fences, release stores) or a random later point, so data
+published without a drain is seen late or out of order, as in-flight stores on the GPU can be; a
+planted missing drain is shown to be caught. What the host cannot show is the GPU's cache
+behaviour itself (scopes, write-back, invalidation)."""
 import os
 import shutil
 import subprocess
@@ -37,16 +41,19 @@ def write_graph(path, off, nbr):
         f.write(np.ascontiguousarray(nbr, np.uint32).tobytes())
 
 
-def run(emu, *args):
-    r = subprocess.run([emu, *map(str, args)], capture_output=True, text=True, timeout=240)
+WEAK = dict(os.environ, KCEMU_WEAK="1")
+
+
+def run(emu, *args, env=None):
+    r = subprocess.run([emu, *map(str, args)], capture_output=True, text=True, timeout=240, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout
 
 
-def peel(emu, tmp_path, off, nbr, variant, *extra):
+def peel(emu, tmp_path, off, nbr, variant, *extra, env=None):
     gp, op = tmp_path / "g.bin", tmp_path / "o.bin"
     write_graph(gp, off, nbr)
-    info = run(emu, "peel", variant, gp, op, *extra)
+    info = run(emu, "peel", variant, gp, op, *extra, env=env)
     d = np.fromfile(op, np.uint32)
     V = len(off) - 1
     kmax, err = int(d[0]), int(d[1])
@@ -133,7 +140,7 @@ def test_prefetch_variant(emu, tmp_path):
     check(off, nbr, *peel(emu, tmp_path, off, nbr, "prefetch")[:5])
 
 
-def test_batch_kernel(emu, tmp_path):
+def test_batch_kernel(emu, tmp_path, env=None):
     paths, refs = [], []
     cases = [c for c in GRAPHS if c[0] in ("K5", "cycle7", "star15", "isolated", "er3000", "chung_lu8000",
                                             "hub3000", "rmat12", "grid3x4", "union")]
@@ -144,7 +151,7 @@ def test_batch_kernel(emu, tmp_path):
         paths.append(p)
         refs.append(O.bz(off, nbr))
     out = tmp_path / "bo.bin"
-    run(emu, "batch", *paths, out)
+    run(emu, "batch", *paths, out, env=env)
     d = np.fromfile(out, np.uint32)
     pos = 0
     for (name, off, nbr), (ref, km) in zip(cases, refs):
@@ -154,7 +161,7 @@ def test_batch_kernel(emu, tmp_path):
         pos += 2 + V
 
 
-def test_serve_kernel(emu, tmp_path):
+def test_serve_kernel(emu, tmp_path, env=None):
     cases = [c for c in GRAPHS if c[0] in ("K12", "er3000", "rmat12", "hub3000", "isolated", "cycle16",
                                             "chung_lu8000", "K3,5", "path9")]
     paths, refs = [], []
@@ -169,7 +176,7 @@ def test_serve_kernel(emu, tmp_path):
     bad = tmp_path / "bad.bin"
     write_graph(bad, off, nbr)
     out = tmp_path / "so.bin"
-    run(emu, "serve", *paths[:4], bad, *paths[4:], out)
+    run(emu, "serve", *paths[:4], bad, *paths[4:], out, env=env)
     d = np.fromfile(out, np.uint32)
     pos = 0
     order = list(range(4)) + [None] + list(range(4, len(cases)))
@@ -188,3 +195,89 @@ def test_serve_kernel(emu, tmp_path):
 
 def test_serve_lease(emu):
     assert "expired flag 1" in run(emu, "lease")
+
+
+# ---- weak memory -------------------------------------------------------------------------
+
+WEAK_CASES = [c for c in GRAPHS if c[0] in ("K12", "er5000", "rmat12", "hub3000", "chung_lu8000", "union")]
+
+
+@pytest.mark.parametrize("variant", ["persistent", "async", "prefetch"])
+@pytest.mark.parametrize("case", WEAK_CASES, ids=[c[0] for c in WEAK_CASES])
+def test_weak_memory_single_graph(emu, tmp_path, variant, case):
+    name, off, nbr = case
+    check(off, nbr, *peel(emu, tmp_path, off, nbr, variant, env=WEAK)[:5])
+
+
+@pytest.mark.parametrize("settings", [
+    dict(heavy=16, chunk=64, flags=0, grid=3, ratio=1),
+    dict(heavy=64, chunk=512, flags=1024, grid=5, ratio=4),
+])
+def test_weak_memory_async_settings(emu, tmp_path, settings):
+    g = K.Graph.rmat(13, 9)
+    off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
+    res = peel(emu, tmp_path, off, nbr, "async", settings["heavy"], settings["chunk"], settings["flags"],
+               settings["grid"], settings["ratio"], env=WEAK)
+    check(off, nbr, *res[:5])
+
+
+def test_weak_memory_batch(emu, tmp_path):
+    test_batch_kernel(emu, tmp_path, env=WEAK)
+
+
+def test_weak_memory_serve(emu, tmp_path):
+    test_serve_kernel(emu, tmp_path, env=WEAK)
+
+
+def test_weak_memory_stepwise(emu, tmp_path):
+    _, off, nbr = next(c for c in GRAPHS if c[0] == "rmat12")
+    gp, op = tmp_path / "g.bin", tmp_path / "o.bin"
+    write_graph(gp, off, nbr)
+    run(emu, "stepwise", gp, op, 0, env=WEAK)
+    d = np.fromfile(op, np.uint32)
+    V = len(off) - 1
+    check(off, nbr, int(d[0]), int(d[1]), d[2:2 + V], d[2 + V:2 + 2 * V], d[2 + 2 * V:])
+
+
+def test_weak_memory_catches_a_missing_drain(tmp_path):
+    """A copy of the batch kernel without the drain between the select's order[] stores and the
+    barrier after which other work-items read them: correct in the ordinary emulation, wrong in
+    the weak-memory one (the stores are still in flight when they are read)."""
+    root = os.path.dirname(HERE)
+    work = tmp_path / "planted"
+    shutil.copytree(os.path.join(HERE, "emu"), work / "tests" / "emu")
+    shutil.copytree(os.path.join(root, "kcoregpu_amd", "csrc"), work / "kcoregpu_amd" / "csrc",
+                    ignore=shutil.ignore_patterns("*.so", "*.o"))
+    hdr = work / "kcoregpu_amd" / "csrc" / "batch_kernel.h"
+    text = hdr.read_text()
+    drained = "        levels++;\n        drain_memory();\n        __syncthreads();"
+    assert text.count(drained) == 1
+    hdr.write_text(text.replace(drained, "        levels++;\n        __syncthreads();"))
+    exe = work / "emu_planted"
+    cc = CLANG if os.path.exists(CLANG) else "clang++"
+    subprocess.run([cc, "-O1", "-std=c++17", "-pthread", "-I", str(work / "tests" / "emu"), "-o", str(exe),
+                    str(work / "tests" / "emu" / "emu_main.cpp")], check=True)
+    paths, refs = [], []
+    for i, (n, m) in enumerate([(3000, 15000), (5000, 60000), (2000, 30000), (4000, 8000)]):
+        g = K.Graph.erdos_renyi(n, m, i + 1)
+        p = tmp_path / f"b{i}.bin"
+        write_graph(p, g.neighbors_offset, g.neighbors)
+        paths.append(p)
+        refs.append(O.bz(g.neighbors_offset, g.neighbors))
+
+    def mismatches(env):
+        out = tmp_path / "o.bin"
+        r = subprocess.run([str(exe), "batch", *map(str, paths), str(out)], capture_output=True, text=True,
+                           timeout=240, env=env)
+        if r.returncode != 0:
+            return -1  # a planted fault may also crash the run: caught as well
+        d = np.fromfile(out, np.uint32)
+        pos, bad = 0, 0
+        for ref, km in refs:
+            V = len(ref)
+            bad += int(d[pos] != km) + int((d[pos + 2:pos + 2 + V] != ref).sum())
+            pos += 2 + V
+        return bad
+
+    assert mismatches(dict(os.environ, KCEMU_WEAK="0")) == 0
+    assert mismatches(WEAK) != 0
