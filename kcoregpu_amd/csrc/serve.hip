@@ -492,6 +492,9 @@ kc_status kc_serve_submit(kc_server* s, uint64_t req_id, const kc_csr* g, uint32
     } else {
         q.charge = std::max<uint64_t>(q.r.bytes, 4ull * q.arcs + 16ull * q.V);  // 8 m + 16 n
     }
+    // every charge a multiple of 256 bytes: with the arena's first fit from offset 0, every
+    // region then starts on a 256-byte boundary (graph_upload_into requires it)
+    q.charge = serve_align(q.charge);
     if (q.charge > s->cfg.hbm_budget_bytes) return KC_ERR_CAPACITY;
     q.t_submit = now_s();
     (q.large ? s->lqueue : s->queue).push_back(q);
