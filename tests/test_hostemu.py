@@ -88,25 +88,53 @@ def test_rewrite_launches():
                    "  kcemu_launch(2048, 256, st, copy_kernel, reinterpret_cast<const uint4*>(g->d), f(x, y), 3);\n")
 
 
+CC = CLANG if os.path.exists(CLANG) else "clang++"
+
+
 @pytest.fixture(scope="module")
-def host(tmp_path_factory):
+def emu_lib(tmp_path_factory):
+    """The library's host sources, launches rewritten, compiled against the stand-in runtime."""
     d = tmp_path_factory.mktemp("hostemu")
     csrc = d / "kcoregpu_amd" / "csrc"
     shutil.copytree(os.path.join(ROOT, "kcoregpu_amd", "csrc"), csrc, ignore=shutil.ignore_patterns("*.so", "*.o"))
     shutil.copytree(os.path.join(ROOT, "include"), d / "include")
-    srcs = []
+    objs = []
     for f in ("kcore_hip.hip", "serve.hip", "tune.cpp"):
         p = csrc / f
         text, n = rewrite_launches(p.read_text())
         assert "<<<" not in text
         p.write_text(text)
-        srcs.append(str(p))
+        o = str(p) + ".o"
+        subprocess.run([CC, "-O1", "-std=c++17", "-fPIC", "-pthread", "-I", os.path.join(HERE, "hostemu"), "-x", "c++",
+                        "-c", str(p), "-o", o], check=True)
+        objs.append(o)
+    return d, objs
+
+
+@pytest.fixture(scope="module")
+def host(emu_lib):
+    d, objs = emu_lib
     exe = d / "host"
-    cc = CLANG if os.path.exists(CLANG) else "clang++"
-    subprocess.run([cc, "-O1", "-std=c++17", "-pthread", "-Wno-unused-command-line-argument",
-                    "-I", os.path.join(HERE, "hostemu"), "-x", "c++", *srcs,
-                    os.path.join(HERE, "hostemu", "host_main.cpp"), "-o", str(exe)], check=True)
+    subprocess.run([CC, "-O1", "-std=c++17", "-pthread", os.path.join(HERE, "hostemu", "host_main.cpp"), *objs,
+                    "-o", str(exe)], check=True)
     return str(exe)
+
+
+@pytest.fixture(scope="module")
+def tools(emu_lib):
+    """The `kcore` driver and the `kcore_serve` load generator, linked with the emulated library."""
+    d, objs = emu_lib
+    csrc = d / "kcoregpu_amd" / "csrc"
+    host_objs = {}
+    for f in ("graph.cpp", "generators.cpp", "graph_capi.cpp", "bincache.cpp", "kcore_main.cpp", "kcore_serve.cpp"):
+        o = str(csrc / f) + ".o"
+        subprocess.run(["g++", "-O1", "-std=c++17", "-fopenmp", "-c", str(csrc / f), "-o", o], check=True)
+        host_objs[f] = o
+    lib = [host_objs[f] for f in ("graph.cpp", "generators.cpp", "graph_capi.cpp", "bincache.cpp")]
+    kcore, serve = str(d / "kcore"), str(d / "kcore_serve")
+    subprocess.run(["g++", "-fopenmp", "-pthread", host_objs["kcore_main.cpp"], *lib, *objs, "-o", kcore], check=True)
+    subprocess.run(["g++", "-fopenmp", "-pthread", host_objs["kcore_serve.cpp"], *lib, *objs, "-o", serve], check=True)
+    return kcore, serve
 
 
 def run(host, *args, timeout=600):
@@ -228,3 +256,82 @@ def test_tuner_c_abi(host, tmp_path):
     assert kmax == km and (d[6:] == ref).all()
     line = cache.read_text().strip().split("\n")
     assert len(line) == 1 and line[0].startswith(f"peel {len(off) - 1} {len(nbr)} ")
+
+
+# the reference's labels (main.cu:60-88), "SM:" with two spaces
+LABELS = ["Ours: ", "SM:  ", "VP: ", "BC: ", "BC + SM: ", "BC + VP: ", "EC: ", "EC + SM: ", "EC + VP: "]
+
+
+def test_driver_end_to_end(tools, tmp_path):
+    """`kcore <edgelist>`: the reference's stdout lines, the text and binary caches, and every output
+    file (raw, text and JSON core numbers, peel order, shell offsets) against the oracle, through
+    the default, cascade-queue and reference-shaped variants."""
+    import re
+    kcore, _ = tools
+    g = K.Graph.erdos_renyi(2500, 12000, 7)
+    g.write_edgelist(str(tmp_path / "er.txt"))
+    os.makedirs(tmp_path / "output")
+    core, kmax = O.bz(g.neighbors_offset, g.neighbors)
+
+    def drive(*args):
+        r = subprocess.run([kcore, *args], cwd=tmp_path, capture_output=True, text=True, timeout=600, env=ENV)
+        assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+        return r.stdout.split("\n")
+
+    out = drive("er.txt", "--core-out", "core.u32", "--core-text", "core.txt", "--json", "--order-out", "order.u32")
+    assert out[:3] == ["Graph loading Started... ", "serialized-er.txt : File couldn't open", "Reading normal file... "]
+    assert re.fullmatch(r"File Loaded in: \d+", out[3])
+    assert out[4] == "******************  er.txt ****************** "
+    assert out[5] == f"V: {g.V}" and out[6] == f"E: {g.arcs}"
+    i = 7
+    for k, lab in enumerate(LABELS):
+        assert re.fullmatch(re.escape(lab) + r"\d+ ", out[i]), out[i]
+        i += 1
+        if k == 0:
+            assert out[i] == f"Kmax: {kmax}"
+            i += 1
+    assert out[i] == "Deallocated... " and out[i + 1] == ""
+    got = np.fromfile(tmp_path / "core.u32", np.uint32)
+    assert (got == core).all()
+    assert (tmp_path / "core.txt").read_text() == "".join(f"{c}\n" for c in core)
+    order = np.fromfile(tmp_path / "order.u32", np.uint32)
+    assert O.check_certificate(g.neighbors_offset, g.neighbors, got, order) == 0
+    shells = np.fromfile(tmp_path / "order.u32.shells", np.uint32)
+    assert shells[0] == 0 and shells[-1] == g.V and len(shells) == kmax + 2
+    want = "{ " + "".join(f'"{v}": {c}, \n' for v, c in enumerate(core) if c)
+    want = want[:-3] + " }\n" if want != "{ " else want
+    assert (tmp_path / "output" / "pkc-kcore-er.txt").read_text() == want
+
+    for variant, cache_args, cache_line in (("async", [], "Reading serialized file... "),
+                                            ("stepwise", ["--binary-cache"], "Reading normal file... "),
+                                            ("auto", ["--binary-cache"], "Reading binary cache... ")):
+        out = drive("er.txt", "--variant", variant, *cache_args, "--core-out", f"{variant}.u32")
+        assert out[:2] == ["Graph loading Started... ", cache_line], (variant, out[:3])
+        assert (np.fromfile(tmp_path / f"{variant}.u32", np.uint32) == core).all(), variant
+        assert f"Kmax: {kmax}" in out, variant
+
+
+@pytest.mark.parametrize("mode", ["served", "plain"])
+def test_kcore_serve_end_to_end(tools, tmp_path, mode):
+    """`kcore_serve`, closed-loop, every result checked against the oracle's core numbers of the
+    request pool (the file the benchmark's CPU leg writes); served with lane requests and a small
+    arena."""
+    import json
+    _, serve = tools
+    pool, seed, median, mmin, mmax = 10, 3, 3000.0, 200, 30000
+    with open(tmp_path / "expect.u32", "wb") as f:
+        for j in range(pool):
+            g = K.Graph.request(seed, j, median_m=median, sigma=1.0, mmin=mmin, mmax=mmax)
+            f.write(O.bz(g.neighbors_offset, g.neighbors)[0].tobytes())
+    args = [serve, "--mode", mode, "--closed", "30", "--pool", str(pool), "--seed", str(seed), "--median", str(median),
+            "--mmin", str(mmin), "--mmax", str(mmax), "--expect", str(tmp_path / "expect.u32"), "--seconds", "1",
+            "--drain-limit", "300"]
+    if mode == "served":
+        args += ["--budget-gb", "0.02", "--workgroups", "2", "--slots", "8", "--large-arcs", "20000",
+                 "--large-lanes", "2"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=900, env=ENV)
+    assert r.returncode == 0, (r.returncode, r.stdout[-1000:], r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().split("\n")[-1])
+    assert res["requests"] == 30 and res["completed"] == 30 and res["unfinished"] == 0
+    assert res["mismatches"] == 0 and res["device_error"] == 0 and res["dropped"] == 0
+    assert res["checked_against"] == "expected core numbers"
