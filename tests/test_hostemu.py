@@ -335,3 +335,17 @@ def test_kcore_serve_end_to_end(tools, tmp_path, mode):
     assert res["requests"] == 30 and res["completed"] == 30 and res["unfinished"] == 0
     assert res["mismatches"] == 0 and res["device_error"] == 0 and res["dropped"] == 0
     assert res["checked_against"] == "expected core numbers"
+
+
+def test_python_api_over_the_cpu_build(emu_lib, tmp_path):
+    """The package's Engine / DeviceGraph / DeviceBatch / Server / tuner wrappers over the CPU build
+    of the library (its path swapped in a separate process; the package has no switch for it):
+    ctypes structures and calls, pinned result buffers released by the garbage collector while the
+    serving kernel is resident, every result against the oracle."""
+    d, objs = emu_lib
+    so = d / "libkcore_hip_emu.so"
+    subprocess.run([CC, "-shared", "-pthread", "-Wl,-Bsymbolic", *objs, "-o", str(so)], check=True)
+    r = subprocess.run(["python3", os.path.join(HERE, "hostemu", "py_api.py"), str(so), str(tmp_path)],
+                       capture_output=True, text=True, timeout=1200, env=ENV)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-3000:])
+    assert r.stdout.strip().endswith("ok")
