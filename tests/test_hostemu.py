@@ -2,8 +2,9 @@
 include/kcore.h — built for the CPU against a stand-in HIP runtime (tests/hostemu/hip: in-order
 stream threads, events, stream capture and graph replay, and hipFree / hipHostFree that wait for
 every stream as HIP's do), with every kernel run by the wave64 emulator (tests/emu). Only the
-`kernel<<<grid, block, shmem, stream>>>(args)` launches are rewritten, into a call that enqueues the
-emulated launch; nothing else in the sources changes. Results must equal the oracle's.
+`kernel<<<grid, block, shmem, stream>>>(args)` launches are rewritten (tests/hostemu/cpu_lib.py),
+into a call that enqueues the emulated launch; nothing else in the sources changes. Results must
+equal the oracle's.
 
 This exercises what the device-code emulation alone cannot: argument set-up, buffer sizes and
 alignment, queue and slot bookkeeping of the serving loop, the large-request lanes, the deferred
@@ -12,6 +13,7 @@ and replay, the tuner and its cache."""
 import os
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -23,60 +25,13 @@ from test_emu import CLANG, write_graph
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(HERE, "hostemu"))
+import cpu_lib  # noqa: E402
+
+rewrite_launches = cpu_lib.rewrite_launches
 pytestmark = pytest.mark.skipif(not os.path.exists(CLANG) and shutil.which("clang++") is None,
                                 reason="needs clang++ (ext_vector_type)")
 ENV = dict(os.environ, KCEMU_WALL_KHZ="10000000", KCEMU_MAX_GRID="4")
-
-
-def _split_top(s):
-    parts, depth, cur = [], 0, ""
-    for ch in s:
-        if ch == "(":
-            depth += 1
-        elif ch == ")":
-            depth -= 1
-        if ch == "," and depth == 0:
-            parts.append(cur.strip())
-            cur = ""
-        else:
-            cur += ch
-    parts.append(cur.strip())
-    return parts
-
-
-def rewrite_launches(text):
-    """kernel<...><<<grid, block, shmem, stream>>>(args);  ->  kcemu_launch(grid, block, stream, kernel<...>, args);"""
-    out, i, n = [], 0, 0
-    while True:
-        k = text.find("<<<", i)
-        if k < 0:
-            out.append(text[i:])
-            return "".join(out), n
-        j = k
-        if text[j - 1] == ">":  # template arguments of the kernel
-            depth, j = 0, j - 1
-            while True:
-                depth += {">": 1, "<": -1}.get(text[j], 0)
-                if depth == 0:
-                    break
-                j -= 1
-        m = j
-        while m > 0 and (text[m - 1].isalnum() or text[m - 1] in "_:"):
-            m -= 1
-        e = text.index(">>>", k)
-        cfg = _split_top(text[k + 3:e])
-        assert len(cfg) == 4 and text[e + 3] == "(", text[m:e + 10]
-        depth, p = 0, e + 3
-        while True:
-            depth += {"(": 1, ")": -1}.get(text[p], 0)
-            if depth == 0:
-                break
-            p += 1
-        args = text[e + 4:p].strip()
-        out.append(text[i:m])
-        out.append(f"kcemu_launch({cfg[0]}, {cfg[1]}, {cfg[3]}, {text[m:k]}" + (f", {args})" if args else ")"))
-        i = p + 1
-        n += 1
 
 
 def test_rewrite_launches():
@@ -88,27 +43,14 @@ def test_rewrite_launches():
                    "  kcemu_launch(2048, 256, st, copy_kernel, reinterpret_cast<const uint4*>(g->d), f(x, y), 3);\n")
 
 
-CC = CLANG if os.path.exists(CLANG) else "clang++"
+CC = cpu_lib.CC
 
 
 @pytest.fixture(scope="module")
 def emu_lib(tmp_path_factory):
     """The library's host sources, launches rewritten, compiled against the stand-in runtime."""
     d = tmp_path_factory.mktemp("hostemu")
-    csrc = d / "kcoregpu_amd" / "csrc"
-    shutil.copytree(os.path.join(ROOT, "kcoregpu_amd", "csrc"), csrc, ignore=shutil.ignore_patterns("*.so", "*.o"))
-    shutil.copytree(os.path.join(ROOT, "include"), d / "include")
-    objs = []
-    for f in ("kcore_hip.hip", "serve.hip", "tune.cpp"):
-        p = csrc / f
-        text, n = rewrite_launches(p.read_text())
-        assert "<<<" not in text
-        p.write_text(text)
-        o = str(p) + ".o"
-        subprocess.run([CC, "-O1", "-std=c++17", "-fPIC", "-pthread", "-I", os.path.join(HERE, "hostemu"), "-x", "c++",
-                        "-c", str(p), "-o", o], check=True)
-        objs.append(o)
-    return d, objs
+    return d, cpu_lib.build_objects(str(d))
 
 
 @pytest.fixture(scope="module")
@@ -343,9 +285,8 @@ def test_python_api_over_the_cpu_build(emu_lib, tmp_path):
     ctypes structures and calls, pinned result buffers released by the garbage collector while the
     serving kernel is resident, every result against the oracle."""
     d, objs = emu_lib
-    so = d / "libkcore_hip_emu.so"
-    subprocess.run([CC, "-shared", "-pthread", "-Wl,-Bsymbolic", *objs, "-o", str(so)], check=True)
-    r = subprocess.run(["python3", os.path.join(HERE, "hostemu", "py_api.py"), str(so), str(tmp_path)],
+    so = cpu_lib.build_shared(objs, str(d / "libkcore_hip_emu.so"))
+    r = subprocess.run(["python3", os.path.join(HERE, "hostemu", "py_api.py"), so, str(tmp_path)],
                        capture_output=True, text=True, timeout=1200, env=ENV)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-3000:])
     assert r.stdout.strip().endswith("ok")
