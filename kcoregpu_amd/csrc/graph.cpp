@@ -1,10 +1,14 @@
 // graph.cpp — edge list -> CSR loader, serialized cache and core writers with the reference's
-// semantics (src/graph.cpp:9-161), built for large inputs: the text is parsed in parallel
-// chunks and the CSR is built by a counting scatter plus per-vertex sort instead of one
-// std::set per vertex. The output arrays are identical to the reference loader's.
+// semantics (src/graph.cpp:9-161), built for large inputs: the text is read and parsed in
+// parallel chunks straight into the pair arrays, and the CSR is built by a bucket partition,
+// a counting sort per bucket and a sort per vertex instead of one std::set per vertex. The
+// output arrays are identical to the reference loader's.
 #include "graph.h"
 
+#include <fcntl.h>
 #include <omp.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cctype>
@@ -28,6 +32,13 @@ std::string basename_of(const std::string& path) {
     return p == std::string::npos ? path : path.substr(p + 1);
 }
 
+// The pair list becomes a CSR in three passes over cache-friendly ranges, without atomics:
+//  1. each thread histograms its share of the arcs (both directions of each pair) by vertex
+//     bucket (V / 2^shift <= 4096 buckets of 2^shift consecutive vertices);
+//  2. each thread writes its arcs into the bucket-major order (every thread owns a slice of each
+//     bucket, so no two threads write the same place);
+//  3. buckets in parallel: a counting sort by source inside the bucket, then per vertex a sort
+//     and duplicate removal (the reference's std::set gives ascending unique lists).
 int build_csr(const uint32_t* s, const uint32_t* t, uint64_t np, int nthreads, Csr& out) {
     const int nt = threads_or_default(nthreads);
     // V = 1 + max id over non-loop pairs (reference src/graph.cpp:89-96)
@@ -41,45 +52,89 @@ int build_csr(const uint32_t* s, const uint32_t* t, uint64_t np, int nthreads, C
     }
     if (any && vmax == UINT32_MAX) return ERR_RANGE;  // V would wrap to 0
     const uint32_t V = any ? vmax + 1 : 1;
+    int shift = 0;
+    while ((((uint64_t)V - 1) >> shift) >= 4096) ++shift;
+    const uint32_t B = (uint32_t)(((uint64_t)V - 1) >> shift) + 1;
+    const uint32_t span = 1u << shift;
+    auto lo = [&](int k) { return (int64_t)(np * (uint64_t)k / (uint64_t)nt); };
 
-    // counting scatter of both directions
-    std::vector<uint64_t> start((size_t)V + 1, 0);
+    // 1. per-thread bucket histograms
+    std::vector<uint64_t> hist((size_t)nt * B, 0);
+#pragma omp parallel num_threads(nt)
     {
-        std::vector<uint32_t> cnt(V, 0);
-        uint32_t* c = cnt.data();
-#pragma omp parallel for num_threads(nt) schedule(static)
-        for (int64_t i = 0; i < (int64_t)np; ++i) {
-            if (s[i] == t[i]) continue;
-            __atomic_fetch_add(&c[s[i]], 1u, __ATOMIC_RELAXED);
-            __atomic_fetch_add(&c[t[i]], 1u, __ATOMIC_RELAXED);
-        }
-        for (uint32_t v = 0; v < V; ++v) start[v + 1] = start[v] + cnt[v];
-    }
-    const uint64_t total = start[V];
-    std::vector<uint32_t> buf(total);
-    {
-        std::vector<uint64_t> cur(start.begin(), start.end() - 1);
-        uint64_t* cp = cur.data();
-        uint32_t* bp = buf.data();
-#pragma omp parallel for num_threads(nt) schedule(static)
-        for (int64_t i = 0; i < (int64_t)np; ++i) {
-            uint32_t a = s[i], b = t[i];
+        const int k = omp_get_thread_num();
+        uint64_t* h = hist.data() + (size_t)k * B;
+        for (int64_t i = lo(k); i < lo(k + 1); ++i) {
+            const uint32_t a = s[i], b = t[i];
             if (a == b) continue;
-            bp[__atomic_fetch_add(&cp[a], 1ull, __ATOMIC_RELAXED)] = b;
-            bp[__atomic_fetch_add(&cp[b], 1ull, __ATOMIC_RELAXED)] = a;
+            ++h[a >> shift];
+            ++h[b >> shift];
         }
     }
-    // per-vertex sort + unique (the reference's std::set gives ascending unique lists)
-    out.V = V;
-    out.degrees.assign(V, 0);
-    uint32_t* deg = out.degrees.data();
-#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024)
-    for (int64_t v = 0; v < (int64_t)V; ++v) {
-        uint32_t* b = buf.data() + start[v];
-        uint32_t* e = buf.data() + start[v + 1];
-        std::sort(b, e);
-        deg[v] = (uint32_t)(std::unique(b, e) - b);
+    // bucket starts, and each thread's first slot in each bucket
+    std::vector<uint64_t> bstart((size_t)B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+        uint64_t acc = bstart[b];
+        for (int k = 0; k < nt; ++k) {
+            const uint64_t c = hist[(size_t)k * B + b];
+            hist[(size_t)k * B + b] = acc;
+            acc += c;
+        }
+        bstart[b + 1] = acc;
     }
+    const uint64_t total = bstart[B];
+
+    // 2. arcs into bucket-major order
+    std::unique_ptr<uint32_t[]> ks(new uint32_t[total + 1]), kd(new uint32_t[total + 1]);
+#pragma omp parallel num_threads(nt)
+    {
+        const int k = omp_get_thread_num();
+        uint64_t* cur = hist.data() + (size_t)k * B;
+        for (int64_t i = lo(k); i < lo(k + 1); ++i) {
+            const uint32_t a = s[i], b = t[i];
+            if (a == b) continue;
+            uint64_t p = cur[a >> shift]++;
+            ks[p] = a;
+            kd[p] = b;
+            p = cur[b >> shift]++;
+            ks[p] = b;
+            kd[p] = a;
+        }
+    }
+
+    // 3. per bucket: counting sort by source, then sorted unique lists per vertex
+    std::unique_ptr<uint32_t[]> buf(new uint32_t[total + 1]);
+    std::unique_ptr<uint64_t[]> start(new uint64_t[(size_t)V + 1]);
+    out.V = V;
+    out.degrees.resize(V);
+    uint32_t* deg = out.degrees.data();
+#pragma omp parallel num_threads(nt)
+    {
+        std::vector<uint64_t> pos(span);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t b = 0; b < (int64_t)B; ++b) {
+            const uint32_t v0 = (uint32_t)b << shift;
+            const uint32_t nv = (uint32_t)std::min<uint64_t>(span, (uint64_t)V - v0);
+            std::fill(pos.begin(), pos.begin() + nv, 0);
+            for (uint64_t p = bstart[b]; p < bstart[b + 1]; ++p) ++pos[ks[p] - v0];
+            uint64_t acc = bstart[b];
+            for (uint32_t j = 0; j < nv; ++j) {
+                const uint64_t c = pos[j];
+                start[v0 + j] = acc;
+                pos[j] = acc;
+                acc += c;
+            }
+            for (uint64_t p = bstart[b]; p < bstart[b + 1]; ++p) buf[pos[ks[p] - v0]++] = kd[p];
+            for (uint32_t j = 0; j < nv; ++j) {
+                uint32_t* lb = buf.get() + start[v0 + j];
+                uint32_t* le = buf.get() + pos[j];
+                std::sort(lb, le);
+                deg[v0 + j] = (uint32_t)(std::unique(lb, le) - lb);
+            }
+        }
+    }
+    ks.reset();
+    kd.reset();
     uint64_t arcs = 0;
     out.offsets.resize((size_t)V + 1);
     out.offsets[0] = 0;
@@ -97,7 +152,7 @@ int build_csr(const uint32_t* s, const uint32_t* t, uint64_t np, int nthreads, C
     const uint32_t* off = out.offsets.data();
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 4096)
     for (int64_t v = 0; v < (int64_t)V; ++v)
-        if (deg[v]) std::memcpy(nb + off[v], buf.data() + start[v], (size_t)deg[v] * 4);
+        if (deg[v]) std::memcpy(nb + off[v], buf.get() + start[v], (size_t)deg[v] * 4);
     return OK;
 }
 
@@ -139,27 +194,42 @@ static inline int fast_line(const char* p, const char* e, uint32_t& s, uint32_t&
     return 1;
 }
 
-static int read_whole(const std::string& path, std::string& data) {
-    FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) return ERR_IO;
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    data.resize(sz > 0 ? (size_t)sz : 0);
-    size_t got = sz > 0 ? std::fread(&data[0], 1, (size_t)sz, f) : 0;
-    std::fclose(f);
-    if (got != data.size()) return ERR_IO;
-    return OK;
+// The whole file in one buffer (no zero fill), read by nt threads with pread.
+static int read_whole(const std::string& path, int nt, std::unique_ptr<char[]>& data, size_t* size) {
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return ERR_IO;
+    struct stat st;
+    if (::fstat(fd, &st) != 0) {
+        ::close(fd);
+        return ERR_IO;
+    }
+    const size_t n = st.st_size > 0 ? (size_t)st.st_size : 0;
+    data.reset(new char[n + 1]);
+    bool bad = false;
+#pragma omp parallel num_threads(nt) reduction(|| : bad)
+    {
+        const int k = omp_get_thread_num(), m = omp_get_num_threads();
+        size_t p = n * (size_t)k / m;
+        const size_t e = n * (size_t)(k + 1) / m;
+        while (p < e && !bad) {
+            const ssize_t r = ::pread(fd, data.get() + p, e - p, (off_t)p);
+            if (r <= 0) bad = true;
+            else p += (size_t)r;
+        }
+    }
+    ::close(fd);
+    data[n] = '\n';
+    *size = n;
+    return bad ? ERR_IO : OK;
 }
 
-int parse_edgelist(const std::string& path, int nthreads, std::vector<uint32_t>& S,
-                   std::vector<uint32_t>& T) {
-    std::string data;
-    int st = read_whole(path, data);
-    if (st) return st;
+int parse_edgelist(const std::string& path, int nthreads, u32vec& S, u32vec& T) {
+    std::unique_ptr<char[]> buf;
+    size_t n = 0;
     const int nt = threads_or_default(nthreads);
-    const char* base = data.data();
-    const size_t n = data.size();
+    int st = read_whole(path, nt, buf, &n);
+    if (st) return st;
+    const char* base = buf.get();
     // chunk boundaries at line starts
     std::vector<size_t> cut(nt + 1, n);
     cut[0] = 0;
@@ -168,57 +238,58 @@ int parse_edgelist(const std::string& path, int nthreads, std::vector<uint32_t>&
         while (p < n && p > 0 && base[p - 1] != '\n') ++p;
         cut[k] = p;
     }
-    std::vector<std::vector<uint32_t>> ls(nt), lt(nt);
+    // lines per chunk (a last line without '\n' counts too): every line gets one pair slot, so
+    // each chunk writes straight to its place; a comment line's slot holds the self loop (0, 0),
+    // which build_csr drops like any other self loop
+    std::vector<uint64_t> first(nt + 1, 0);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int k = 0; k < nt; ++k) {
+        const char* p = base + cut[k];
+        const char* e = base + cut[k + 1];
+        uint64_t c = (uint64_t)std::count(p, e, '\n');
+        if (e > p && e[-1] != '\n') ++c;
+        first[k + 1] = c;
+    }
+    for (int k = 0; k < nt; ++k) first[k + 1] += first[k];
+    S.resize(first[nt]);
+    T.resize(first[nt]);
+    uint32_t* sp = S.data();
+    uint32_t* tp = T.data();
     int slow = 0;
-#pragma omp parallel num_threads(nt) reduction(| : slow)
-    {
-        int k = omp_get_thread_num();
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(| : slow)
+    for (int k = 0; k < nt; ++k) {
         const char* p = base + cut[k];
         const char* end = base + cut[k + 1];
-        auto& vs = ls[k];
-        auto& vt = lt[k];
-        vs.reserve((end - p) / 12 + 16);
-        vt.reserve((end - p) / 12 + 16);
-        while (p < end && !slow) {
+        uint64_t i = first[k];
+        while (p < end) {
             const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
             const char* le = nl ? nl : end;
-            uint32_t a, b;
-            int r = fast_line(p, le, a, b);
+            uint32_t a = 0, b = 0;
+            const int r = fast_line(p, le, a, b);
             if (r < 0) {
                 slow = 1;
                 break;
             }
-            if (r == 1 && a != b) {
-                vs.push_back(a);
-                vt.push_back(b);
-            }
+            sp[i] = a;  // (0, 0) for a comment line
+            tp[i] = b;
+            ++i;
             p = nl ? nl + 1 : end;
         }
     }
-    S.clear();
-    T.clear();
-    if (!slow) {
-        size_t tot = 0;
-        for (int k = 0; k < nt; ++k) tot += ls[k].size();
-        S.reserve(tot);
-        T.reserve(tot);
-        for (int k = 0; k < nt; ++k) {
-            S.insert(S.end(), ls[k].begin(), ls[k].end());
-            T.insert(T.end(), lt[k].begin(), lt[k].end());
-        }
-        return OK;
-    }
+    if (!slow) return OK;
     // exact sequential emulation, line for line. The reference leaves s and t
     // uninitialised before the first line; they start at 0 here.
+    S.clear();
+    T.clear();
     unsigned int s = 0, t = 0;
     size_t p = 0;
     std::string line;
     while (p < n) {
-        size_t q = data.find('\n', p);
-        size_t le = q == std::string::npos ? n : q;
+        const char* q = (const char*)std::memchr(base + p, '\n', n - p);
+        const size_t le = q ? (size_t)(q - base) : n;
         line.assign(base + p, le - p);
-        p = q == std::string::npos ? n : q + 1;
-        if (std::isalpha(line[0])) continue;
+        p = q ? le + 1 : n;
+        if (std::isalpha((unsigned char)line[0])) continue;
         std::istringstream iss(line);
         iss >> s >> t;
         if (s == t) continue;
@@ -229,7 +300,7 @@ int parse_edgelist(const std::string& path, int nthreads, std::vector<uint32_t>&
 }
 
 int load_edgelist(const std::string& path, int nthreads, Csr& out) {
-    std::vector<uint32_t> s, t;
+    u32vec s, t;
     int st = parse_edgelist(path, nthreads, s, t);
     if (st) return st;
     return build_csr(s.data(), t.data(), s.size(), nthreads, out);
@@ -311,10 +382,11 @@ struct TokenReader {
 }  // namespace
 
 int read_serialized(const std::string& path, Csr& g) {
-    std::string data;
-    int st = read_whole(path, data);
+    std::unique_ptr<char[]> data;
+    size_t n = 0;
+    int st = read_whole(path, omp_get_max_threads(), data, &n);
     if (st) return st;
-    TokenReader r{data.data(), data.data() + data.size()};
+    TokenReader r{data.get(), data.get() + n};
     uint32_t V, E;
     if (!r.next(V) || !r.next(E)) return ERR_FORMAT;
     g.V = V;

@@ -8,10 +8,35 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace kcg {
+
+// Allocator whose resize() leaves new elements uninitialised: the loader's arrays are written
+// in parallel right after they are sized, so a serial zero fill would only cost time.
+template <class T>
+struct uninit_alloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = uninit_alloc<U>;
+    };
+    uninit_alloc() = default;
+    template <class U>
+    uninit_alloc(const uninit_alloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+using u32vec = std::vector<uint32_t, uninit_alloc<uint32_t>>;
 
 // A CSR graph as the loader builds it: V vertices, `arcs` = 2 * undirected edges,
 // adjacency ascending and duplicate-free, no self loops (reference src/graph.cpp:97-122).
@@ -19,9 +44,9 @@ struct Csr {
     uint32_t V = 0;
     uint64_t arcs = 0;
     uint32_t dmax = 0;
-    std::vector<uint32_t> offsets;    // V + 1
-    std::vector<uint32_t> neighbors;  // arcs
-    std::vector<uint32_t> degrees;    // V
+    u32vec offsets;    // V + 1
+    u32vec neighbors;  // arcs
+    u32vec degrees;    // V
 };
 
 // Status codes shared with the C ABI.
@@ -34,10 +59,9 @@ int build_csr(const uint32_t* s, const uint32_t* t, uint64_t npairs, int nthread
 
 // Parse a text edge list with the reference's per-line rules (src/graph.cpp:68-96):
 // lines whose first char is alphabetic are skipped, each other line is read as
-// `iss >> s >> t` (values kept from the previous line when extraction fails), self loops
-// are dropped. Appends the kept pairs.
-int parse_edgelist(const std::string& path, int nthreads, std::vector<uint32_t>& s,
-                   std::vector<uint32_t>& t);
+// `iss >> s >> t` (values kept from the previous line when extraction fails). Returns the
+// pairs, which may include self loops (s == t: build_csr drops them, as the reference does).
+int parse_edgelist(const std::string& path, int nthreads, u32vec& s, u32vec& t);
 
 int load_edgelist(const std::string& path, int nthreads, Csr& out);
 int read_serialized(const std::string& path, Csr& out);
