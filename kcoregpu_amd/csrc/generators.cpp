@@ -119,7 +119,7 @@ int gen_chung_lu(uint32_t n, double gamma, double wmin, double wmax, double avg,
     const uint64_t M = (uint64_t)std::llround((double)n * avg / 2.0);
     const uint64_t key = stream_key(seed, 2);
     std::vector<uint32_t> perm = permutation(n, stream_key(seed, 3));
-    std::vector<uint32_t> S(M), T(M);
+    u32vec S(M), T(M);  // written in parallel below: no zero fill
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = 0; i < (int64_t)M; ++i) {
         uint64_t x = bounded(rnd(key, 2 * (uint64_t)i), tot);
@@ -144,7 +144,7 @@ int gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c, uint
     const uint32_t nv = 1u << scale;
     std::vector<uint32_t> perm = permutation(nv, stream_key(seed, 5));
     const uint32_t words = (scale + 1) / 2;  // two 32-bit quadrant draws per 64-bit value
-    std::vector<uint32_t> S(ndraws), T(ndraws);
+    u32vec S(ndraws), T(ndraws);  // written in parallel below: no zero fill
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = 0; i < (int64_t)ndraws; ++i) {
         uint32_t u = 0, v = 0;
