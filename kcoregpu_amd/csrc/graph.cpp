@@ -317,32 +317,49 @@ static void put_u32(std::string& buf, uint32_t v, char sep) {
     buf.push_back(sep);
 }
 
+// Writes n values, each followed by `sep`, formatted by nt threads in rounds of nt slices (so
+// the memory held is bounded) and written in order. Each thread formats into its own buffer
+// through a local cursor: nothing it writes shares a cache line with another thread's writes.
+struct FmtBuf {
+    std::unique_ptr<char[]> p;
+    size_t len = 0;
+    char pad[64 - sizeof(std::unique_ptr<char[]>) - sizeof(size_t)];
+};
+constexpr uint64_t kFmtSlice = 1u << 20;  // values per thread and round (at most 11 bytes each)
+
+static bool write_values(FILE* f, const uint32_t* a, uint64_t n, char sep, int nt, std::vector<FmtBuf>& bufs) {
+    bool ok = true;
+    for (uint64_t r0 = 0; r0 < n; r0 += kFmtSlice * (uint64_t)nt) {
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+        for (int k = 0; k < nt; ++k) {
+            char* q = bufs[k].p.get();
+            const uint64_t lo = std::min(n, r0 + kFmtSlice * (uint64_t)k);
+            const uint64_t hi = std::min(n, lo + kFmtSlice);
+            for (uint64_t i = lo; i < hi; ++i) {
+                q = std::to_chars(q, q + 10, a[i]).ptr;
+                *q++ = sep;
+            }
+            bufs[k].len = (size_t)(q - bufs[k].p.get());
+        }
+        for (int k = 0; k < nt; ++k) ok = ok && std::fwrite(bufs[k].p.get(), 1, bufs[k].len, f) == bufs[k].len;
+    }
+    return ok;
+}
+
 int write_serialized(const Csr& g, const std::string& path) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return ERR_IO;
-    std::string buf;
-    buf.reserve(1 << 22);
-    bool short_write = false;
-    auto flush = [&]() {
-        short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
-        buf.clear();
-    };
-    put_u32(buf, g.V, '\n');
-    put_u32(buf, (uint32_t)g.arcs, '\n');
-    for (uint32_t i = 0; i < g.V; ++i) {
-        put_u32(buf, g.degrees[i], '\n');
-        if (buf.size() > (1 << 22) - 64) flush();
-    }
-    for (uint64_t i = 0; i <= g.V; ++i) {
-        put_u32(buf, g.offsets[i], ' ');
-        if (buf.size() > (1 << 22) - 64) flush();
-    }
-    for (uint64_t i = 0; i < g.arcs; ++i) {
-        put_u32(buf, g.neighbors[i], ' ');
-        if (buf.size() > (1 << 22) - 64) flush();
-    }
-    flush();
-    return (std::fclose(f) == 0 && !short_write) ? OK : ERR_IO;
+    const int nt = threads_or_default(0);
+    std::vector<FmtBuf> bufs(nt);
+    for (auto& b : bufs) b.p.reset(new char[kFmtSlice * 11]);
+    std::string head;
+    put_u32(head, g.V, '\n');
+    put_u32(head, (uint32_t)g.arcs, '\n');
+    bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+    ok = ok && write_values(f, g.degrees.data(), g.V, '\n', nt, bufs);
+    ok = ok && write_values(f, g.offsets.data(), (uint64_t)g.V + 1, ' ', nt, bufs);
+    ok = ok && write_values(f, g.neighbors.data(), g.arcs, ' ', nt, bufs);
+    return (std::fclose(f) == 0 && ok) ? OK : ERR_IO;
 }
 
 // One core number per line, each followed by '\n': the form of the degree block of the
@@ -381,19 +398,9 @@ struct TokenReader {
 };
 }  // namespace
 
-int read_serialized(const std::string& path, Csr& g) {
-    std::unique_ptr<char[]> data;
-    size_t n = 0;
-    int st = read_whole(path, omp_get_max_threads(), data, &n);
-    if (st) return st;
-    TokenReader r{data.get(), data.get() + n};
-    uint32_t V, E;
-    if (!r.next(V) || !r.next(E)) return ERR_FORMAT;
-    g.V = V;
-    g.arcs = E;
-    g.degrees.resize(V);
-    g.offsets.resize((size_t)V + 1);
-    g.neighbors.resize(E);
+// The serial reader defines the format: whitespace-separated decimal tokens, V, E, V degrees,
+// V + 1 offsets, E neighbours, anything after them ignored (reference src/graph.cpp:9-33).
+static int read_serialized_serial(TokenReader r, uint32_t V, uint32_t E, Csr& g) {
     g.dmax = 0;
     for (uint32_t i = 0; i < V; ++i) {
         if (!r.next(g.degrees[i])) return ERR_FORMAT;
@@ -403,6 +410,89 @@ int read_serialized(const std::string& path, Csr& g) {
         if (!r.next(g.offsets[i])) return ERR_FORMAT;
     for (uint64_t i = 0; i < E; ++i)
         if (!r.next(g.neighbors[i])) return ERR_FORMAT;
+    return OK;
+}
+
+int read_serialized(const std::string& path, Csr& g) {
+    std::unique_ptr<char[]> data;
+    size_t n = 0;
+    const int nt = omp_get_max_threads();
+    int st = read_whole(path, nt, data, &n);
+    if (st) return st;
+    TokenReader r{data.get(), data.get() + n};
+    uint32_t V, E;
+    if (!r.next(V) || !r.next(E)) return ERR_FORMAT;
+    g.V = V;
+    g.arcs = E;
+    g.degrees.resize(V);
+    g.offsets.resize((size_t)V + 1);
+    g.neighbors.resize(E);
+    // Parallel path: the rest cut at whitespace into nt chunks, tokens counted per chunk, then
+    // each chunk parses its tokens into their places. Any token among the needed ones that is not
+    // plain digits within uint32, or too few tokens, hands the file to the serial reader, which
+    // decides (it accepts, e.g., garbage glued to the very last needed token).
+    const char* b0 = r.p;
+    const char* e0 = r.e;
+    const uint64_t need = 2ull * V + 1 + E;
+    std::vector<const char*> cut(nt + 1, e0);
+    cut[0] = b0;
+    for (int k = 1; k < nt; ++k) {
+        const char* p = std::max(cut[k - 1], b0 + (size_t)(e0 - b0) * k / nt);
+        while (p < e0 && !is_ws(*p)) ++p;
+        cut[k] = p;
+    }
+    std::vector<uint64_t> first(nt + 1, 0);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int k = 0; k < nt; ++k) {
+        uint64_t c = 0;
+        bool in = false;
+        for (const char* p = cut[k]; p < cut[k + 1]; ++p) {
+            const bool ws = is_ws(*p);
+            c += (!ws && !in) ? 1 : 0;
+            in = !ws;
+        }
+        first[k + 1] = c;
+    }
+    for (int k = 0; k < nt; ++k) first[k + 1] += first[k];
+    bool irregular = first[nt] < need;
+    uint32_t dmax = 0;
+    uint32_t* deg = g.degrees.data();
+    uint32_t* off = g.offsets.data();
+    uint32_t* nbr = g.neighbors.data();
+    if (!irregular) {
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(|| : irregular) reduction(max : dmax)
+        for (int k = 0; k < nt; ++k) {
+            uint64_t idx = first[k];
+            const char* p = cut[k];
+            const char* e = cut[k + 1];
+            while (idx < need) {
+                while (p < e && is_ws(*p)) ++p;
+                if (p >= e) break;
+                uint64_t a = 0;
+                int nd = 0;
+                while (p < e && *p >= '0' && *p <= '9' && nd < 11) {
+                    a = a * 10 + (uint64_t)(*p++ - '0');
+                    ++nd;
+                }
+                if (nd == 0 || nd > 10 || a > UINT32_MAX || (p < e && !is_ws(*p))) {
+                    irregular = true;
+                    break;
+                }
+                const uint32_t v = (uint32_t)a;
+                if (idx < V) {
+                    deg[idx] = v;
+                    dmax = std::max(dmax, v);
+                } else if (idx <= 2ull * V) {
+                    off[idx - V] = v;
+                } else {
+                    nbr[idx - 2ull * V - 1] = v;
+                }
+                ++idx;
+            }
+        }
+    }
+    if (irregular) return read_serialized_serial(r, V, E, g);
+    g.dmax = dmax;
     return OK;
 }
 

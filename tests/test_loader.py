@@ -186,3 +186,83 @@ def test_driver_error_lines_match_reference(tmp_path):
     assert r.returncode == 255
     assert r.stdout == ("Graph loading Started... \nserialized-missing.txt : File couldn't open\n"
                         "Reading normal file... \nload graph file failed \n")
+
+
+def _serial_reader(data: bytes):
+    """The serialized-cache reader's definition (src/graph.cpp:9-33 reads with `>>`): whitespace-
+    separated decimal tokens V, E, V degrees, V + 1 offsets, E neighbours; a needed token must start
+    with a digit and fit in uint32; whatever follows the last needed token is not read.
+    Returns (degrees, offsets, neighbours) or None for a malformed file."""
+    pos, n = 0, len(data)
+    ws = b" \t\n\v\f\r"
+
+    def nxt():
+        nonlocal pos
+        while pos < n and data[pos] in ws:
+            pos += 1
+        q = pos
+        while pos < n and 48 <= data[pos] <= 57:
+            pos += 1
+        if pos == q or int(data[q:pos]) > 0xFFFFFFFF:
+            return None
+        return int(data[q:pos])
+
+    V, E = nxt(), None
+    if V is not None:
+        E = nxt()
+    if V is None or E is None:
+        return None
+    out = []
+    for count in (V, V + 1, E):
+        vals = []
+        for _ in range(count):
+            v = nxt()
+            if v is None:
+                return None
+            vals.append(v)
+        out.append(vals)
+    return out
+
+
+def test_serialized_reader_irregular_files(tmp_path):
+    """The reader parses large regular caches in parallel and hands anything irregular to the serial
+    reader: both must give exactly the definition's result on mutated caches (other whitespace,
+    garbage glued to tokens, truncation, oversized values, trailing text)."""
+    import random
+    g = K.Graph.rmat(11, 7)
+    p = tmp_path / "ser"
+    g.write_serialized(str(p))
+    base = p.read_bytes()
+    rng = random.Random(5)
+    seps = [b" ", b"\t", b"\n", b"  ", b"\r\n", b"\v", b"\f", b" \t "]
+    for case in range(60):
+        toks = base.split()
+        kind = case % 6
+        if kind == 0:    # other whitespace between tokens
+            data = toks[0]
+            for t in toks[1:]:
+                data += rng.choice(seps) + t
+        elif kind == 1:  # garbage glued to one token (accepted only on the last needed one)
+            i = rng.choice([len(toks) - 1, rng.randrange(2, len(toks))])
+            toks[i] = toks[i] + b"x"
+            data = b" ".join(toks)
+        elif kind == 2:  # truncated
+            data = b" ".join(toks[:rng.randrange(2, len(toks))])
+        elif kind == 3:  # a value beyond uint32
+            toks[rng.randrange(2, len(toks))] = b"4294967296"
+            data = b" ".join(toks)
+        elif kind == 4:  # trailing text after the last needed token
+            data = base + b"trailing words 12 x"
+        else:            # a non-numeric token
+            toks[rng.randrange(2, len(toks))] = b"-3"
+            data = b" ".join(toks)
+        q = tmp_path / f"case{case}"
+        q.write_bytes(data)
+        want = _serial_reader(data)
+        if want is None:
+            with pytest.raises(K.KCoreError):
+                K.Graph.read_serialized(str(q))
+            continue
+        h = K.Graph.read_serialized(str(q))
+        assert list(h.degrees) == want[0] and list(h.neighbors_offset) == want[1], case
+        assert list(h.neighbors) == want[2], case
