@@ -11,6 +11,8 @@
 // In weak-memory mode (KCEMU_WEAK=1) each work-item has a store buffer (see hip/hip_runtime.h):
 // at every yield each buffered store completes with probability 1/3 (in random order); a sleep,
 // the end of the work-item and a pass in which nothing could run complete them all.
+// KCEMU_JITTER=N: after a scheduling pass, a workgroup pauses for up to 200 us with probability
+// 1/N, so workgroups overtake each other at random points (more cross-workgroup interleavings).
 // Deadlocks inside a workgroup (every live fiber blocked on something that cannot complete)
 // abort with a message; waits on other workgroups are the kernels' own, bounded by their wall
 // clock limits (clock_ticks() counts 10 ns ticks).
@@ -60,6 +62,10 @@ inline thread_local Group* g_grp = nullptr;
 inline const bool g_weak = [] {
     const char* e = std::getenv("KCEMU_WEAK");
     return e && *e && *e != '0';
+}();
+inline const uint32_t g_jitter = [] {
+    const char* e = std::getenv("KCEMU_JITTER");
+    return e && *e ? (uint32_t)std::strtoul(e, nullptr, 10) : 0u;
 }();
 
 inline void sb_complete(const SbEntry& e) {
@@ -264,6 +270,7 @@ inline void trampoline(uint32_t hi, uint32_t lo) {
 
 inline void run_group(Group& g) {
     g_grp = &g;
+    uint64_t jitter_state = 0x2545F4914F6CDD1Dull ^ ((uint64_t)g.bid * 0x9E3779B97F4A7C15ull) ^ (uint64_t)clock_ticks();
     const size_t stack = 128 * 1024;
     for (uint32_t t = 0; t < g.block; ++t) {
         Fiber& f = g.f[t];
@@ -339,6 +346,13 @@ inline void run_group(Group& g) {
             // nothing could run: every buffered store completes (the GPU's do, eventually)
             for (uint32_t t = 0; t < g.block; ++t) sb_drain_fiber(g.f[t]);
             std::this_thread::yield();
+        }
+        if (g_jitter) {
+            uint64_t& r = jitter_state;
+            r ^= r << 13;
+            r ^= r >> 7;
+            r ^= r << 17;
+            if (r % g_jitter == 0) std::this_thread::sleep_for(std::chrono::microseconds((r >> 20) % 200));
         }
         (void)any_live;
     }

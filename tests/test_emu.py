@@ -281,3 +281,13 @@ def test_weak_memory_catches_a_missing_drain(tmp_path):
 
     assert mismatches(dict(os.environ, KCEMU_WEAK="0")) == 0
     assert mismatches(WEAK) != 0
+
+
+@pytest.mark.parametrize("variant", ["persistent", "async"])
+def test_weak_memory_with_jitter(emu, tmp_path, variant):
+    """Weak memory plus random pauses of whole workgroups (KCEMU_JITTER): workgroups overtake each
+    other at random points of the claim, termination and barrier protocols."""
+    env = dict(WEAK, KCEMU_JITTER="20")
+    for name in ("rmat12", "hub3000", "chung_lu8000"):
+        _, off, nbr = next(c for c in GRAPHS if c[0] == name)
+        check(off, nbr, *peel(emu, tmp_path, off, nbr, variant, 16, 64, 0, 5, 2, env=env)[:5])
