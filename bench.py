@@ -143,7 +143,7 @@ def bench_single(args, K, eng):
         threads = host_threads()
         O.levelsync(g.neighbors_offset, g.neighbors, nthreads=threads)  # warm
         runs = []
-        for _ in range(3):
+        for _ in range(5):
             tc = time.perf_counter()
             res = O.levelsync(g.neighbors_offset, g.neighbors, nthreads=threads)
             runs.append(time.perf_counter() - tc)
@@ -154,7 +154,7 @@ def bench_single(args, K, eng):
         bz_s = time.perf_counter() - tb
         cpu = {"value": m / cpu_s, "unit": "edges/s", "cores": threads, "kind": "port",
                "sample": f"the full R-MAT scale {args.scale} graph: level-synchronous restatement of the "
-                         f"reference peel on {threads} threads, median of 3 after a warm-up "
+                         f"reference peel on {threads} threads, median of 5 after a warm-up "
                          f"({cpu_s:.2f} s, kmax {res['kmax']}); GPU core numbers "
                          f"{'identical' if same else 'DIFFERENT'}",
                "gpu_matches": same and bool((bz_core == core).all()),
