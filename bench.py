@@ -129,6 +129,13 @@ def bench_single(args, K, eng):
     k_avg_s = float(np.mean(kernel_ms)) / 1e3
     achieved = st["bytes_alg"] / k_avg_s / 1e9
     core = dg.core()
+    # the box's own HBM ceiling beside the published peak: the degree-copy kernel over 2 x 1 GiB
+    try:
+        copy_gbps = eng.copy_bandwidth(1 << 30, 10)
+        log(f"[single] copy-kernel ceiling {copy_gbps:.0f} GB/s")
+    except Exception as e:  # noqa: BLE001 - reported as null
+        log(f"[single] copy-kernel ceiling failed: {e!r}")
+        copy_gbps = None
     # the reference's own shape on the same GPU: host level loop over every level 0..kmax, one
     # select and per-cascade process launches per level, frontier size read back each time
     rs = dg.peel("stepwise", flags=K.FLAG_NO_LEVEL_SKIP, download=True)
@@ -170,6 +177,8 @@ def bench_single(args, K, eng):
                    "grid": st["grid"], "participants": st["participants"], "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": profiled_traffic(args, tuned),
+                     "copy_ceiling": copy_gbps,
+                     "frac_of_copy_ceiling": achieved / copy_gbps if copy_gbps else None,
                      "kernel": KERNEL_OF.get(tuned["variant"], "peel_persistent"),
                      "kernel_ms": float(np.mean(kernel_ms)),
                      "bytes_alg": st["bytes_alg"], "bytes_scan": st["bytes_scan"],

@@ -268,6 +268,10 @@ kc_status kc_graph_download_debug(kc_ctx* ctx, const kc_graph* g, uint32_t* word
                                   uint32_t* n_out, uint32_t* ctrl_out, uint32_t ctrl_cap);
 /* Workgroups of the peel kernel the runtime reports resident per CU, and the CU count. */
 kc_status kc_occupancy(kc_ctx* ctx, uint32_t block, int variant, uint32_t* per_cu, uint32_t* cus);
+/* Empirical HBM ceiling: the copy kernel that initialises the working degrees (16-byte loads and
+ * stores, grid-stride), run `reps` times over two fresh buffers of `bytes` each; *gbps = bytes read
+ * plus bytes written per second (10^9), from device events. The buffers are freed before return. */
+kc_status kc_copy_bandwidth(kc_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -347,6 +347,13 @@ class Engine:
     def selftest(self) -> None:
         _check(N.hip().kc_selftest(self._c), "kc_selftest")
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Empirical HBM ceiling in GB/s: the degree-copy kernel over two buffers of nbytes
+        (bytes read plus bytes written per second)."""
+        g = C.c_double()
+        _check(N.hip().kc_copy_bandwidth(self._c, nbytes, reps, C.byref(g)), "kc_copy_bandwidth")
+        return g.value
+
     def upload(self, g: Graph) -> "DeviceGraph":
         h = N.kc_csr(g.V, g.arcs, _u32(g.neighbors_offset),
                      _u32(g.neighbors) if g.arcs else N.u32p(), _u32(g.degrees))

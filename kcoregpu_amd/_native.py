@@ -159,6 +159,7 @@ def hip() -> C.CDLL:
         L.kc_wall_clock_khz.argtypes = [vp]
         L.kc_wall_clock_khz.restype = C.c_uint64
         L.kc_occupancy.argtypes = [vp, C.c_uint32, C.c_int, u32p, u32p]
+        L.kc_copy_bandwidth.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         L.kc_batch_upload.argtypes = [vp, C.POINTER(kc_batch_csr), pp]
         L.kc_batch_free.argtypes = [vp]
         L.kc_batch_free.restype = None

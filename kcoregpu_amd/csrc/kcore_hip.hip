@@ -664,6 +664,34 @@ kc_status kc_occupancy(kc_ctx* c, uint32_t block, int variant, uint32_t* per_cu,
     return KC_OK;
 }
 
+kc_status kc_copy_bandwidth(kc_ctx* c, uint64_t bytes, uint32_t reps, double* gbps) {
+    if (!c || !gbps || bytes < 16 || reps == 0) return KC_ERR_ARG;
+    const uint64_t nvec = bytes / 16;
+    if (nvec > 0xFFFFFFFFull) return KC_ERR_CAPACITY;
+    KC_TRY(hipSetDevice(c->device));
+    uint4 *src = nullptr, *dst = nullptr;
+    KC_TRY(hipMalloc(&src, nvec * 16));
+    if (hipMalloc(&dst, nvec * 16) != hipSuccess) {
+        lib_free_device(src);
+        return KC_ERR_OOM;
+    }
+    const uint32_t grid = (uint32_t)c->cus * 16;
+    fill_kernel<uint32_t><<<1024, 256, 0, c->stream>>>(reinterpret_cast<uint32_t*>(src), 7u, nvec * 4);
+    copy_kernel<<<grid, 256, 0, c->stream>>>(src, dst, (uint32_t)nvec);  // warm-up
+    hipError_t e = hipEventRecord(c->ev0, c->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) copy_kernel<<<grid, 256, 0, c->stream>>>(src, dst, (uint32_t)nvec);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    lib_free_device(src);
+    lib_free_device(dst);
+    KC_TRY(e);
+    *gbps = ms > 0 ? 2.0 * (double)nvec * 16.0 * reps / (ms * 1e-3) / 1e9 : 0.0;
+    return KC_OK;
+}
+
 kc_status kc_selftest(kc_ctx* c) {
     if (!c) return KC_ERR_ARG;
     KC_TRY(hipSetDevice(c->device));

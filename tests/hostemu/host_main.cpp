@@ -66,6 +66,9 @@ int run_peel(char** argv) {
     FILE* out = std::fopen(argv[3], "wb");
     kc_ctx* c = open_ctx();
     CHECK(kc_selftest(c) == KC_OK);
+    double gbps = 0;
+    CHECK(kc_copy_bandwidth(c, 1u << 20, 2, &gbps) == KC_OK && gbps > 0);
+    CHECK(kc_copy_bandwidth(c, 8, 2, &gbps) == KC_ERR_ARG);
     kc_csr h{g.V, g.arcs, g.off.data(), g.nbr.data(), nullptr};
     kc_graph* dg = nullptr;
     CHECK(kc_graph_upload(c, &h, &dg) == KC_OK);

@@ -44,6 +44,12 @@ def check_against(g, res, core, kmax):
             assert (r.core[order[shells[k]:shells[k + 1]]] == k).all()
 
 
+def test_copy_ceiling(engine):
+    gbps = engine.copy_bandwidth(256 << 20, 5)
+    say(f"copy kernel: {gbps:.0f} GB/s")
+    assert 100 < gbps < 20000  # a plausible HBM copy rate, read + write
+
+
 def test_selftest(engine):
     engine.selftest()
 
