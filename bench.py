@@ -210,6 +210,9 @@ def bench_batch(args, K, eng):
     for _ in range(args.batch_steps):
         _, _, st = db.peel(block=tuned["block"], flags=flags, download=False)
         times.append(st["ms_peel"])
+    # B_alg summed over the graphs: per graph V_g entries scanned per level it ran, 16 n + 20 m
+    b_alg = st["bytes_alg"]
+    achieved = b_alg / (float(np.mean(times)) / 1e3) / 1e9
     wall = (time.perf_counter() - t_start) / args.batch_steps
     core, kmax, _ = db.peel(block=tuned["block"], flags=flags)
     log(f"[batch] {wall * 1e3:.2f} ms per batch peel ({G / wall:.0f} graphs/s)")
@@ -232,6 +235,8 @@ def bench_batch(args, K, eng):
     out = {"graphs": G, "vertices": b.total_vertices, "arcs": b.total_arcs,
            "graphs_per_s": G / wall, "ms_per_batch": wall * 1e3,
            "edges_per_s": b.total_arcs / 2 / wall, "kernel_ms_mean": float(np.mean(times)),
+           "bytes_alg": b_alg, "bytes_scan": st["bytes_scan"], "bytes_edge": st["bytes_edge"],
+           "achieved_gbps": achieved, "frac_of_peak": achieved / HBM_PEAK_GBPS,
            "plain_graphs_per_s": G / plain_s, "speedup_vs_plain": (G / wall) / (G / plain_s),
            "bit_exact_vs_plain": mism == 0, "mismatched_graphs": mism,
            "block": tuned["block"], "hip_graph": True,

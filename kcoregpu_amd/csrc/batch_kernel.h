@@ -32,7 +32,8 @@ struct BatchArgs {
     uint32_t* order;          // packed per-graph peel order
     uint32_t* kmax;           // G
     const uint32_t* sched;    // G graph ids, in claim order
-    uint32_t* ctrl;           // [0] next claim, [64] levels, [128] rounds, [192] error
+    uint32_t* ctrl;           // [0] next claim, [64] levels, [128] rounds, [192] error,
+                              // [224..225] degree entries scanned (uint64)
     uint32_t G;
     uint32_t flags;
 };
@@ -252,6 +253,9 @@ __device__ void batch_peel_graph(const BatchArgs& a, BatchState<BLOCK>& S, uint3
         st_rlx(a.kmax + g, km);
         add_rlx(a.ctrl + 64, levels);
         add_rlx(a.ctrl + 128, rounds);
+        // degree entries scanned: V_g per level this graph ran (words 224-225, one uint64)
+        __hip_atomic_fetch_add(reinterpret_cast<uint64_t*>(a.ctrl + 224), (uint64_t)levels * Vg, __ATOMIC_RELAXED,
+                               KC_AGENT);
         if (err) __hip_atomic_fetch_max(a.ctrl + 192, err, __ATOMIC_RELAXED, KC_AGENT);
     }
 }

@@ -923,8 +923,10 @@ kc_status kc_batch_peel(kc_ctx* c, kc_batch* b, const kc_params* p, uint32_t* co
         stt->subrounds = ctl[128];
         stt->block = block;
         // scan term: each graph scans its V_g entries once per level it runs
+        stt->scanned_entries = (uint64_t)ctl[224] | ((uint64_t)ctl[225] << 32);
+        stt->bytes_scan = 4 * stt->scanned_entries;
         stt->bytes_edge = 16ull * b->nv + 20ull * (b->na / 2);
-        stt->bytes_alg = stt->bytes_edge;  // the per-graph scan term is added by the caller
+        stt->bytes_alg = stt->bytes_scan + stt->bytes_edge;
     }
     return KC_OK;
 }

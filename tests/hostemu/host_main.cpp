@@ -133,9 +133,14 @@ int run_batch(int argc, char** argv) {
     kc_batch_csr hb{Gn, vo[Gn], ao[Gn], vo.data(), ao.data(), off.data(), nbr.data(), nullptr};
     kc_batch* db = nullptr;
     CHECK(kc_batch_upload(c, &hb, &db) == KC_OK);
+    uint64_t scanned = 0;
     auto emit = [&](const kc_params& p) {
         std::vector<uint32_t> core(vo[Gn], 0xEEEEEEEEu), kmax(Gn, 0xEEEEEEEEu);
-        CHECK(kc_batch_peel(c, db, &p, core.data(), kmax.data(), nullptr) == KC_OK);
+        kc_stats st{};
+        CHECK(kc_batch_peel(c, db, &p, core.data(), kmax.data(), &st) == KC_OK);
+        CHECK(st.bytes_alg == st.bytes_scan + st.bytes_edge && st.bytes_scan == 4 * st.scanned_entries);
+        if (scanned) CHECK(st.scanned_entries == scanned);  // the same levels every time
+        scanned = st.scanned_entries;
         for (uint32_t i = 0; i < Gn; ++i) {
             put(out, kmax[i]);
             put(out, core.data() + vo[i], gs[i].V);
@@ -164,6 +169,9 @@ int run_batch(int argc, char** argv) {
     kc_batch_free(db);
     kc_close(c);
     std::fclose(out);
+    FILE* sf = std::fopen((std::string(argv[2]) + ".scan").c_str(), "wb");
+    std::fwrite(&scanned, 8, 1, sf);
+    std::fclose(sf);
     return 0;
 }
 

@@ -143,6 +143,10 @@ def test_batch_c_abi(host, tmp_path):
     out = tmp_path / "bo.bin"
     run(host, "batch", out, *paths)
     d = np.fromfile(out, np.uint32)
+    # degree entries scanned: every graph at least once, at most once per level 0..kmax
+    scanned = int(np.fromfile(str(out) + ".scan", np.uint64)[0])
+    sizes = [(len(off) - 1, km) for (_, off, _), (_, km) in zip(GRAPHS, refs)]
+    assert sum(V for V, _ in sizes) <= scanned <= sum(V * (km + 1) for V, km in sizes)
     pos = 0
     for rnd in ("eager", "replay 1", "replay 2", "replay 3", "tuned"):
         for (name, off, nbr), (ref, km) in zip(GRAPHS, refs):
