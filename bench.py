@@ -1,8 +1,8 @@
 """Benchmark of the k-core engine on one MI355X: single-graph, batched and served peeling.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--no-tune]
-                    [--batch-graphs 4096] [--serve-seconds 10] [--only single|batch|serve]
-                    [--no-cpu-baseline]
+                    [--batch-graphs 4096] [--serve-seconds 10] [--serve-rate R] [--serve-sweep a,b,..]
+                    [--only single|batch|serve] [--no-cpu-baseline]
 
 Prints one JSON line. Its headline (metric/value) is the single-graph peel of an R-MAT scale-24
 graph; "batched" and "served" hold the other two workloads.
@@ -21,7 +21,9 @@ batched 4096 graphs, n log-uniform in [1K, 200K], mean degree uniform in [4, 32]
 served  kcore_serve: open-loop Poisson requests (edge counts lognormal, median 50K, sigma 1.0,
         truncated to [1K, 50M]; R-MAT or Chung-Lu by a seeded coin flip) answered by the
         resident serving kernel and by the plain per-request path (upload, peel, download,
-        free). Every served result is checked against core numbers computed by the oracle.
+        free). The served path's capacity is measured first (all requests at once); the
+        open-loop rate is 0.7 x that capacity unless --serve-rate names one (--serve-sweep adds
+        rates). Every served result is checked against core numbers computed by the oracle.
 cpu_baseline  the CPU level-synchronous restatement of the reference's peel (oracle), run on
         the host threads on the same R-MAT graph; its core numbers are compared with the GPU's.
 """
@@ -264,20 +266,31 @@ def bench_serve(args, K, expect_path):
         common += ["--expect", expect_path]
     lim = int(args.serve_seconds * 3 + 120)
     res = {}
-    for name, extra in (("served", ["--mode", "served", "--rate", str(args.serve_rate)]),
-                        ("served_at_plain_rate", ["--mode", "served", "--rate", str(args.plain_rate)]),
-                        ("plain", ["--mode", "plain", "--rate", str(args.plain_rate),
-                                   "--drain-limit", "30"]),
-                        ("plain_capacity", ["--mode", "plain", "--closed", str(args.plain_closed)])):
+    keys = ("offered_rate", "seconds", "closed", "requests", "completed", "unfinished", "dropped", "overloaded",
+            "graphs_per_s", "edges_per_s", "p50_ms", "p90_ms", "p99_ms", "max_ms", "mismatches", "checked_against")
+
+    def leg(name, extra):
         r = run_serve(common + extra, lim)
         log(f"[serve] {name}: {r['graphs_per_s']:.0f} graphs/s p50 {r['p50_ms']:.2f} ms "
             f"p99 {r['p99_ms']:.2f} ms mismatches {r['mismatches']} unfinished {r['unfinished']}")
-        res[name] = {k: r[k] for k in ("offered_rate", "seconds", "closed", "requests", "completed",
-                                       "unfinished", "dropped", "overloaded", "graphs_per_s",
-                                       "edges_per_s", "p50_ms", "p90_ms", "p99_ms", "max_ms",
-                                       "mismatches", "checked_against")}
+        res[name] = {k: r[k] for k in keys}
+        return r
+
+    # the served path's capacity (all requests at once), then the open-loop stream at 0.7 x that
+    # rate unless --serve-rate names one
+    cap = leg("served_capacity", ["--mode", "served", "--closed", str(args.serve_closed)])
+    rate = args.serve_rate if args.serve_rate else round(0.7 * cap["graphs_per_s"])
+    leg("served", ["--mode", "served", "--rate", str(rate)])
+    leg("served_at_plain_rate", ["--mode", "served", "--rate", str(args.plain_rate)])
+    leg("plain", ["--mode", "plain", "--rate", str(args.plain_rate), "--drain-limit", "30"])
+    leg("plain_capacity", ["--mode", "plain", "--closed", str(args.plain_closed)])
+    for r in args.serve_sweep:
+        leg(f"served_at_{int(r)}", ["--mode", "served", "--rate", str(r)])
+    res["rate_rule"] = ("--serve-rate" if args.serve_rate else "0.7 x served_capacity") + f": {rate}/s"
     res["graphs_per_s_vs_plain_capacity"] = (res["served"]["graphs_per_s"]
                                              / res["plain_capacity"]["graphs_per_s"])
+    res["capacity_vs_plain_capacity"] = (res["served_capacity"]["graphs_per_s"]
+                                         / res["plain_capacity"]["graphs_per_s"])
     res["workload"] = (f"open-loop Poisson, pool of {args.serve_pool} requests (seed "
                        f"{args.serve_seed}), edges lognormal median 50000 sigma 1.0 in [1000, 5e7], "
                        "R-MAT / Chung-Lu by coin flip")
@@ -321,7 +334,12 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-rate", type=float, default=6000.0)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="open-loop arrival rate; 0 = 0.7 x the served path's measured capacity")
+    ap.add_argument("--serve-closed", type=int, default=4000,
+                    help="requests of the closed-loop run that measures the served capacity")
+    ap.add_argument("--serve-sweep", type=lambda v: [float(x) for x in v.split(",") if x], default=[],
+                    help="extra open-loop rates, comma-separated")
     ap.add_argument("--plain-rate", type=float, default=150.0)
     ap.add_argument("--plain-closed", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=256)
