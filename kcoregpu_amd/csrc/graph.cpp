@@ -317,32 +317,45 @@ static void put_u32(std::string& buf, uint32_t v, char sep) {
     buf.push_back(sep);
 }
 
-// Writes n values, each followed by `sep`, formatted by nt threads in rounds of nt slices (so
-// the memory held is bounded) and written in order. Each thread formats into its own buffer
-// through a local cursor: nothing it writes shares a cache line with another thread's writes.
+// Items [0, n) are formatted by nt threads in rounds of nt slices (so the memory held is bounded):
+// fmt(i, q) writes item i (at most `maxb` bytes) at q and returns the end. Each thread formats
+// into its own buffer through a local cursor, so nothing it writes shares a cache line with
+// another thread's writes; the round's buffers are then passed in order to emit(p, len).
 struct FmtBuf {
     std::unique_ptr<char[]> p;
     size_t len = 0;
     char pad[64 - sizeof(std::unique_ptr<char[]>) - sizeof(size_t)];
 };
-constexpr uint64_t kFmtSlice = 1u << 20;  // values per thread and round (at most 11 bytes each)
+constexpr uint64_t kFmtSlice = 1u << 18;  // items per thread and round
 
-static bool write_values(FILE* f, const uint32_t* a, uint64_t n, char sep, int nt, std::vector<FmtBuf>& bufs) {
-    bool ok = true;
+template <class Fmt, class Emit>
+static void format_parallel(uint64_t n, size_t maxb, int nt, Fmt fmt, Emit emit) {
+    std::vector<FmtBuf> bufs(nt);
+    for (auto& b : bufs) b.p.reset(new char[kFmtSlice * maxb]);
     for (uint64_t r0 = 0; r0 < n; r0 += kFmtSlice * (uint64_t)nt) {
 #pragma omp parallel for num_threads(nt) schedule(static, 1)
         for (int k = 0; k < nt; ++k) {
             char* q = bufs[k].p.get();
             const uint64_t lo = std::min(n, r0 + kFmtSlice * (uint64_t)k);
             const uint64_t hi = std::min(n, lo + kFmtSlice);
-            for (uint64_t i = lo; i < hi; ++i) {
-                q = std::to_chars(q, q + 10, a[i]).ptr;
-                *q++ = sep;
-            }
+            for (uint64_t i = lo; i < hi; ++i) q = fmt(i, q);
             bufs[k].len = (size_t)(q - bufs[k].p.get());
         }
-        for (int k = 0; k < nt; ++k) ok = ok && std::fwrite(bufs[k].p.get(), 1, bufs[k].len, f) == bufs[k].len;
+        for (int k = 0; k < nt; ++k) emit(bufs[k].p.get(), bufs[k].len);
     }
+}
+
+// n values, each followed by `sep`
+static bool write_values(FILE* f, const uint32_t* a, uint64_t n, char sep, int nt) {
+    bool ok = true;
+    format_parallel(
+        n, 11, nt,
+        [&](uint64_t i, char* q) {
+            q = std::to_chars(q, q + 10, a[i]).ptr;
+            *q++ = sep;
+            return q;
+        },
+        [&](const char* p, size_t len) { ok = ok && std::fwrite(p, 1, len, f) == len; });
     return ok;
 }
 
@@ -350,15 +363,13 @@ int write_serialized(const Csr& g, const std::string& path) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return ERR_IO;
     const int nt = threads_or_default(0);
-    std::vector<FmtBuf> bufs(nt);
-    for (auto& b : bufs) b.p.reset(new char[kFmtSlice * 11]);
     std::string head;
     put_u32(head, g.V, '\n');
     put_u32(head, (uint32_t)g.arcs, '\n');
     bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
-    ok = ok && write_values(f, g.degrees.data(), g.V, '\n', nt, bufs);
-    ok = ok && write_values(f, g.offsets.data(), (uint64_t)g.V + 1, ' ', nt, bufs);
-    ok = ok && write_values(f, g.neighbors.data(), g.arcs, ' ', nt, bufs);
+    ok = ok && write_values(f, g.degrees.data(), g.V, '\n', nt);
+    ok = ok && write_values(f, g.offsets.data(), (uint64_t)g.V + 1, ' ', nt);
+    ok = ok && write_values(f, g.neighbors.data(), g.arcs, ' ', nt);
     return (std::fclose(f) == 0 && ok) ? OK : ERR_IO;
 }
 
@@ -367,18 +378,8 @@ int write_serialized(const Csr& g, const std::string& path) {
 int write_core_text(const std::string& path, uint32_t V, const uint32_t* core) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return ERR_IO;
-    std::string buf;
-    buf.reserve(1 << 22);
-    bool short_write = false;
-    for (uint32_t i = 0; i < V; ++i) {
-        put_u32(buf, core[i], '\n');
-        if (buf.size() > (1 << 22) - 64) {
-            short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
-            buf.clear();
-        }
-    }
-    short_write |= std::fwrite(buf.data(), 1, buf.size(), f) != buf.size();
-    return (std::fclose(f) == 0 && !short_write) ? OK : ERR_IO;
+    const bool ok = write_values(f, core, V, '\n', threads_or_default(0));
+    return (std::fclose(f) == 0 && ok) ? OK : ERR_IO;
 }
 
 namespace {
@@ -502,32 +503,35 @@ int read_serialized(const std::string& path, Csr& g) {
 int write_kcore_json(const std::string& path, uint32_t V, const uint32_t* core) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return ERR_IO;
-    std::string buf = "{ ";
+    bool ok = std::fwrite("{ ", 1, 2, f) == 2;
+    // every entry ends with ", \n": the last 3 bytes written so far are held back, so that the
+    // final overwrite by " }" happens in memory
+    char carry[3];
     bool any = false;
-    char tmp[48];
-    for (uint64_t i = 0; i < V; ++i) {
-        if (core[i] == 0) continue;
-        any = true;
-        buf.push_back('"');
-        auto r = std::to_chars(tmp, tmp + 24, i);
-        buf.append(tmp, r.ptr);
-        buf.append("\": ");
-        r = std::to_chars(tmp, tmp + 24, core[i]);
-        buf.append(tmp, r.ptr);
-        buf.append(", \n");
-        if (buf.size() > (1 << 22)) {
-            // keep the last 3 bytes buffered so the final overwrite stays in memory
-            std::fwrite(buf.data(), 1, buf.size() - 3, f);
-            buf.erase(0, buf.size() - 3);
-        }
-    }
-    if (any) {
-        size_t k = buf.size() - 3;
-        buf[k] = ' ';
-        buf[k + 1] = '}';
-    }
-    std::fwrite(buf.data(), 1, buf.size(), f);
-    return std::fclose(f) == 0 ? OK : ERR_IO;
+    format_parallel(
+        V, 27, threads_or_default(0),
+        [&](uint64_t i, char* q) {
+            if (core[i] == 0) return q;
+            *q++ = '"';
+            q = std::to_chars(q, q + 10, i).ptr;
+            *q++ = '"';
+            *q++ = ':';
+            *q++ = ' ';
+            q = std::to_chars(q, q + 10, core[i]).ptr;
+            *q++ = ',';
+            *q++ = ' ';
+            *q++ = '\n';
+            return q;
+        },
+        [&](const char* p, size_t len) {
+            if (len == 0) return;
+            if (any) ok = ok && std::fwrite(carry, 1, 3, f) == 3;
+            ok = ok && std::fwrite(p, 1, len - 3, f) == len - 3;
+            std::memcpy(carry, p + len - 3, 3);
+            any = true;
+        });
+    if (any) ok = ok && std::fwrite(" }\n", 1, 3, f) == 3;
+    return (std::fclose(f) == 0 && ok) ? OK : ERR_IO;
 }
 
 int write_core_u32(const std::string& path, uint32_t V, const uint32_t* core) {

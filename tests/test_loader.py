@@ -266,3 +266,20 @@ def test_serialized_reader_irregular_files(tmp_path):
         h = K.Graph.read_serialized(str(q))
         assert list(h.degrees) == want[0] and list(h.neighbors_offset) == want[1], case
         assert list(h.neighbors) == want[2], case
+
+
+def test_core_writers_across_formatting_rounds(tmp_path):
+    """The JSON and text core writers format in parallel rounds of 2^18 vertices per thread; long
+    stretches of zero cores leave whole rounds empty. Bytes equal the reference writer's rules."""
+    rng = np.random.default_rng(3)
+    V = 2_500_000
+    core = np.zeros(V, np.uint32)
+    core[[7, 600_000, 2_400_000]] = [3, 4, 9]
+    core[1_000_000:1_000_300] = rng.integers(1, 50, 300)
+    j, t = tmp_path / "c.json", tmp_path / "c.txt"
+    K.write_kcore_json(str(j), core)
+    nz = np.flatnonzero(core)
+    want = "{ " + "".join(f'"{v}": {core[v]}, \n' for v in nz)
+    assert j.read_text() == want[:-3] + " }\n"
+    K.write_core_text(str(t), core)
+    assert t.read_bytes() == "".join(f"{c}\n" for c in core).encode()
