@@ -48,10 +48,13 @@ G read_graph(const char* path) {
 
 template <class T>
 T* zalloc(size_t n) {
-    // 256-byte aligned, zeroed, like the device allocations (plus slack for quad loads)
+    // 256-byte aligned, zeroed, with slack as device allocations have; KCEMU_EXACT=1: exactly n
+    // elements, so that AddressSanitizer reports any access past the end
+    static const bool exact = std::getenv("KCEMU_EXACT") && *std::getenv("KCEMU_EXACT") == '1';
+    const size_t m = exact ? (n ? n : 1) : n + 64;
     void* p = nullptr;
-    if (posix_memalign(&p, 256, (n + 64) * sizeof(T))) std::exit(4);
-    std::memset(p, 0, (n + 64) * sizeof(T));
+    if (posix_memalign(&p, 256, m * sizeof(T))) std::exit(4);
+    std::memset(p, 0, m * sizeof(T));
     return static_cast<T*>(p);
 }
 

@@ -72,9 +72,9 @@ def rewrite_launches(text):
         n += 1
 
 
-def build_objects(work):
+def build_objects(work, extra=()):
     """Copies the sources into `work`, rewrites the launches and compiles them (position
-    independent). Returns the object files."""
+    independent; `extra`: more compiler flags, e.g. a sanitizer). Returns the object files."""
     csrc = os.path.join(work, "kcoregpu_amd", "csrc")
     shutil.copytree(os.path.join(ROOT, "kcoregpu_amd", "csrc"), csrc, ignore=shutil.ignore_patterns("*.so", "*.o"))
     shutil.copytree(os.path.join(ROOT, "include"), os.path.join(work, "include"))
@@ -87,8 +87,8 @@ def build_objects(work):
         with open(p, "w") as fh:
             fh.write(text)
         o = p + ".o"
-        subprocess.run([CC, "-O1", "-std=c++17", "-fPIC", "-pthread", "-I", HERE, "-x", "c++", "-c", p, "-o", o],
-                       check=True)
+        subprocess.run([CC, "-O1", "-std=c++17", "-fPIC", "-pthread", *extra, "-I", HERE, "-x", "c++", "-c", p,
+                        "-o", o], check=True)
         objs.append(o)
     return objs
 

@@ -191,9 +191,11 @@ inline void device_sync(const char* who, double limit_s) {
     }
 }
 inline double free_wait() { return (double)env_u32("KCEMU_FREE_WAIT_S", 20); }
+// KCEMU_EXACT=1: exactly the bytes asked for (with AddressSanitizer, any access past an
+// allocation's end is reported); otherwise page granularity plus slack, as device memory has.
 inline void* alloc_filled(size_t bytes, size_t align) {
     void* p = nullptr;
-    const size_t n = ((bytes + 4095) & ~(size_t)4095) + 4096;  // page granularity plus slack
+    const size_t n = env_u32("KCEMU_EXACT", 0) ? (bytes ? bytes : 1) : ((bytes + 4095) & ~(size_t)4095) + 4096;
     if (posix_memalign(&p, align, n)) return nullptr;
     std::memset(p, 0xA5, n);
     return p;

@@ -294,3 +294,43 @@ def test_python_api_over_the_cpu_build(emu_lib, tmp_path):
                        capture_output=True, text=True, timeout=1200, env=ENV)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-3000:])
     assert r.stdout.strip().endswith("ok")
+
+
+@pytest.fixture(scope="module")
+def host_asan(tmp_path_factory):
+    """The same harness built with AddressSanitizer (host code and emulated kernels alike)."""
+    d = tmp_path_factory.mktemp("hostemu_asan")
+    flags = ["-fsanitize=address", "-fno-omit-frame-pointer", "-gline-tables-only"]
+    objs = cpu_lib.build_objects(str(d), flags)
+    exe = str(d / "host")
+    subprocess.run([CC, "-O1", "-std=c++17", "-pthread", *flags, os.path.join(HERE, "hostemu", "host_main.cpp"),
+                    *objs, "-o", exe], check=True)
+    return exe
+
+
+ASAN_ENV = dict(ENV, KCEMU_EXACT="1", ASAN_OPTIONS="detect_leaks=0:detect_stack_use_after_return=0")
+
+
+def test_no_out_of_bounds_access(host_asan, tmp_path):
+    """Every device allocation exactly as large as the host code asks (KCEMU_EXACT), under
+    AddressSanitizer: every single-graph variant on a hub graph (chunked arcs) and a served stream
+    with lane requests touch nothing past an allocation's end (on the GPU such an access can fault
+    the device)."""
+    for name in ("hub2500",):
+        _, off, nbr = next(c for c in GRAPHS if c[0] == name)
+        gp, op = tmp_path / "g.bin", tmp_path / "o.bin"
+        write_graph(gp, off, nbr)
+        r = subprocess.run([host_asan, "peel", str(gp), str(op)], capture_output=True, text=True, timeout=600,
+                           env=ASAN_ENV)
+        assert r.returncode == 0, (name, r.stderr[-3000:])
+    gs = [K.Graph.request(43, i, median_m=3000.0, sigma=1.2, mmin=200, mmax=30000) for i in range(8)]
+    paths = []
+    for i, g in enumerate(gs):
+        p = tmp_path / f"s{i}.bin"
+        write_graph(p, g.neighbors_offset, g.neighbors)
+        paths.append(str(p))
+    large = sorted(g.arcs for g in gs)[-2]
+    budget = 3 * max(48 * g.V + 6 * g.arcs + (2 << 20) for g in gs)
+    r = subprocess.run([host_asan, "serve", str(tmp_path / "so.bin"), str(large), str(budget), *paths],
+                       capture_output=True, text=True, timeout=900, env=ASAN_ENV)
+    assert r.returncode == 0, r.stderr[-3000:]
