@@ -298,9 +298,11 @@ def test_python_api_over_the_cpu_build(emu_lib, tmp_path):
 
 @pytest.fixture(scope="module")
 def host_asan(tmp_path_factory):
-    """The same harness built with AddressSanitizer (host code and emulated kernels alike)."""
+    """The same harness built with AddressSanitizer and UndefinedBehaviorSanitizer (host code and
+    emulated kernels alike)."""
     d = tmp_path_factory.mktemp("hostemu_asan")
-    flags = ["-fsanitize=address", "-fno-omit-frame-pointer", "-gline-tables-only"]
+    flags = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer",
+             "-gline-tables-only"]
     objs = cpu_lib.build_objects(str(d), flags)
     exe = str(d / "host")
     subprocess.run([CC, "-O1", "-std=c++17", "-pthread", *flags, os.path.join(HERE, "hostemu", "host_main.cpp"),
@@ -313,7 +315,7 @@ ASAN_ENV = dict(ENV, KCEMU_EXACT="1", ASAN_OPTIONS="detect_leaks=0:detect_stack_
 
 def test_no_out_of_bounds_access(host_asan, tmp_path):
     """Every device allocation exactly as large as the host code asks (KCEMU_EXACT), under
-    AddressSanitizer: every single-graph variant on a hub graph (chunked arcs) and a served stream
+    AddressSanitizer and UndefinedBehaviorSanitizer: every single-graph variant on a hub graph (chunked arcs) and a served stream
     with lane requests touch nothing past an allocation's end (on the GPU such an access can fault
     the device)."""
     for name in ("hub2500",):
