@@ -26,6 +26,10 @@ served  kcore_serve: open-loop Poisson requests (edge counts lognormal, median 5
         rates). Every served result is checked against core numbers computed by the oracle.
 cpu_baseline  the CPU level-synchronous restatement of the reference's peel (oracle), run on
         the host threads on the same R-MAT graph; its core numbers are compared with the GPU's.
+
+The autotuning and the batched leg run in child processes (this script with --leg), started before
+this process touches the GPU, so that a failure there cannot take the headline line with it: the
+headline then uses the default settings and says so (detail.settings_from).
 """
 from __future__ import annotations
 
@@ -90,7 +94,13 @@ def host_threads():
 
 
 # ---------------------------------------------------------------------------------------------
-def bench_single(args, K, eng):
+DEFAULT_PARAMS = {"variant": 2, "block": 512, "heavy_threshold": 64, "chunk_arcs": 512, "lds_depth": 0,
+                  "compact_ratio": 4, "poll_max": 2}
+
+
+def bench_single(args, K, eng, tune_cache=None, tuned_by=None, tune_only=False):
+    """tune_cache: settings tuned by a child process (answered from the cache here); None: the
+    defaults. tune_only: tune, print the settings and return (the child's job)."""
     import numpy as np
     t0 = time.time()
     log(f"[single] generating R-MAT scale {args.scale} ef {args.edge_factor} seed {args.seed}")
@@ -101,14 +111,17 @@ def bench_single(args, K, eng):
     dg = eng.upload(g)
     h2d_s = time.time() - t1
     names = PARAMS
-    if args.no_tune:
-        tuned = {"variant": 2, "block": 512, "heavy_threshold": 64, "chunk_arcs": 512, "lds_depth": 0,
-                 "compact_ratio": 4, "poll_max": 2, "trials": 0, "ms": None}
+    if tune_cache is None:
+        tuned = dict(DEFAULT_PARAMS, trials=0, ms=None)
     else:
         tt = time.time()
-        tuned = dg.tune(reps=2, cache_path=args.tune_cache)
+        tuned = dg.tune(reps=2, cache_path=tune_cache)
+        if tuned_by:
+            tuned["trials"] = tuned_by.get("trials", tuned["trials"])
         log(f"[single] tuned in {time.time() - tt:.1f}s ({tuned['trials']} trials): "
             + " ".join(f"{k}={tuned[k]}" for k in names))
+        if tune_only:
+            return tuned
     prm = {k: tuned[k] for k in names}
 
     def step():
@@ -316,6 +329,23 @@ def serve_expect(args, K, path):
             "sample": f"O(m) bucket peel of the {args.serve_pool} pool graphs ({edges} edges, {t:.2f} s)"}
 
 
+def run_leg(args, leg, limit, tune_cache=None):
+    """Runs this script as a child for one leg; returns (its JSON object, None) or (None, error)."""
+    argv = [sys.executable, os.path.abspath(__file__), *sys.argv[1:], "--leg", leg]
+    if tune_cache:
+        argv += ["--tune-cache", tune_cache]
+    try:
+        p = subprocess.run(argv, stdout=subprocess.PIPE, text=True, timeout=limit)
+    except subprocess.TimeoutExpired:
+        log(f"[{leg}] child process exceeded {limit} s")
+        return None, f"timed out after {limit} s"
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not lines:
+        log(f"[{leg}] child process exited {p.returncode}")
+        return None, f"child process exited {p.returncode}"
+    return json.loads(lines[-1]), None
+
+
 # ---------------------------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -344,6 +374,7 @@ def main():
     ap.add_argument("--plain-closed", type=int, default=600)
     ap.add_argument("--serve-pool", type=int, default=256)
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--leg", choices=("tune", "batch"), default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -353,18 +384,23 @@ def main():
 
     import kcoregpu_amd as K
 
-    eng = K.Engine(0)
+    # Child processes: the tuning and the batched leg. Their failures (a device fault ends a
+    # process) cannot take the headline line with them.
+    if args.leg == "tune":
+        eng = K.Engine(0)
+        tuned = bench_single(args, K, eng, tune_cache=args.tune_cache, tune_only=True)
+        eng.close()
+        print(json.dumps({"tuned": tuned}), flush=True)
+        return
+    if args.leg == "batch":
+        eng = K.Engine(0)
+        res = bench_batch(args, K, eng)
+        eng.close()
+        print(json.dumps({"batched": res}), flush=True)
+        return
+
+    # Every child runs before this process touches the GPU itself.
     single = batch = served = None
-    if args.only in (None, "single"):
-        single = bench_single(args, K, eng)
-    # the batched and served legs report a failure in their own object; the headline line stays
-    if args.only in (None, "batch") and args.batch_graphs > 0:
-        try:
-            batch = bench_batch(args, K, eng)
-        except Exception as e:  # noqa: BLE001 - reported in the JSON line
-            log(f"[batch] failed: {e!r}")
-            batch = {"error": repr(e)}
-    eng.close()
     if args.only in (None, "serve") and args.serve_seconds > 0:
         try:
             with tempfile.TemporaryDirectory() as td:
@@ -378,6 +414,23 @@ def main():
         except Exception as e:  # noqa: BLE001 - reported in the JSON line
             log(f"[serve] failed: {e!r}")
             served = {"error": repr(e)}
+    if args.only in (None, "batch") and args.batch_graphs > 0:
+        res, err = run_leg(args, "batch", 3000)
+        batch = res["batched"] if res else {"error": err}
+    with tempfile.TemporaryDirectory() as td:
+        if args.only in (None, "single"):
+            cache, tuned_by, tune_note = None, None, "defaults (--no-tune)"
+            if not args.no_tune:
+                cache = args.tune_cache or os.path.join(td, "tune.txt")
+                res, err = run_leg(args, "tune", 1500, tune_cache=cache)
+                if res:
+                    tuned_by, tune_note = res["tuned"], "tuned in a child process"
+                else:
+                    cache, tune_note = None, f"defaults: tuning failed in its child process ({err})"
+            eng = K.Engine(0)
+            single = bench_single(args, K, eng, tune_cache=cache, tuned_by=tuned_by)
+            single["detail"]["settings_from"] = tune_note
+            eng.close()
 
     out = {"metric": "peeled edges/s", "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

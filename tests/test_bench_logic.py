@@ -63,3 +63,29 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_child_leg_failures_are_reported(monkeypatch):
+    """The tuning and the batched leg run as child processes: one that dies, prints nothing or
+    overruns its limit comes back as an error string, never as an exception."""
+    args = argparse.Namespace()
+
+    def dead(argv, **kw):
+        assert argv[-2:] == ["--leg", "batch"] or "--tune-cache" in argv
+        return subprocess.CompletedProcess(argv, -11, stdout="partial output\n")
+
+    monkeypatch.setattr(bench.subprocess, "run", dead)
+    assert bench.run_leg(args, "batch", 10) == (None, "child process exited -11")
+
+    def slow(argv, **kw):
+        raise subprocess.TimeoutExpired(argv, kw.get("timeout"))
+
+    monkeypatch.setattr(bench.subprocess, "run", slow)
+    assert bench.run_leg(args, "tune", 5, tune_cache="/x/t.txt") == (None, "timed out after 5 s")
+
+    def ok(argv, **kw):
+        assert argv[argv.index("--tune-cache") + 1] == "/x/t.txt"
+        return subprocess.CompletedProcess(argv, 0, stdout='noise\n{"tuned": {"variant": 2}}\n')
+
+    monkeypatch.setattr(bench.subprocess, "run", ok)
+    assert bench.run_leg(args, "tune", 5, tune_cache="/x/t.txt") == ({"tuned": {"variant": 2}}, None)
