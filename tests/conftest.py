@@ -12,6 +12,29 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: full-size configuration")
+    config.addinivalue_line("markers", "late: GPU test collected after the broad GPU checks")
+
+
+# GPU tests run broadest first: single-graph parity of the sub-round kernel, batches, serving, the
+# driver; then every case of the cascade-queue variant and of the reference-shaped host loop, and
+# tests marked `late`; then any other GPU module (collected last, in name order). A failure still
+# fails the run, but under `pytest -x` it cannot keep the broader checks from running. CPU tests
+# keep their order.
+_GPU_ORDER = ("test_gpu_parity", "test_gpu_batch", "test_gpu_serve", "test_gpu_driver", "test_gpu_async")
+
+
+def pytest_collection_modifyitems(config, items):
+    def rank(item):
+        if item.get_closest_marker("gpu") is None:
+            return (0, 0)
+        mod = item.module.__name__.rsplit(".", 1)[-1] if item.module else ""
+        if mod not in _GPU_ORDER:
+            return (3, 0)
+        late = (mod == "test_gpu_async" or "async" in item.nodeid or "stepwise" in item.nodeid
+                or item.get_closest_marker("late") is not None)
+        return (2 if late else 1, _GPU_ORDER.index(mod))
+
+    items.sort(key=rank)  # stable: the order inside each group is kept
 
 
 @pytest.fixture(scope="session")

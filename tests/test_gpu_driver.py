@@ -60,11 +60,23 @@ def test_driver_lines_cache_and_outputs(tmp_path):
     assert (tmp_path / "output" / "pkc-kcore-er.txt").read_text() == want
 
     # second run: the serialized cache written by the first run is read instead of the text
-    out = run(tmp_path, "er.txt", "--variant", "async", "--core-out", "core2.u32")
+    out = run(tmp_path, "er.txt", "--variant", "lds", "--core-out", "core2.u32")
     assert out[:2] == ["Graph loading Started... ", "Reading serialized file... "]
     assert out[3] == f"V: {g.V}" and out[4] == f"E: {g.arcs}"
-    assert re.fullmatch(r"async: \d+ ", out[5]) and out[6] == f"Kmax: {kmax}"
+    assert re.fullmatch(r"lds: \d+ ", out[5]) and out[6] == f"Kmax: {kmax}"
     assert (np.fromfile(tmp_path / "core2.u32", np.uint32) == core).all()
+
+
+@pytest.mark.parametrize("variant", ["async", "stepwise"])
+def test_driver_other_variants(tmp_path, variant):
+    g = K.Graph.erdos_renyi(3000, 15000, 12)
+    g.write_edgelist(str(tmp_path / "er.txt"))
+    os.makedirs(tmp_path / "output")
+    core, kmax = O.bz(g.neighbors_offset, g.neighbors)
+    out = run(tmp_path, "er.txt", "--variant", variant, "--core-out", "core.u32")
+    assert out[5] == f"V: {g.V}" and out[6] == f"E: {g.arcs}"
+    assert re.fullmatch(re.escape(variant) + r": \d+ ", out[7]) and out[8] == f"Kmax: {kmax}"
+    assert (np.fromfile(tmp_path / "core.u32", np.uint32) == core).all()
 
 
 def test_driver_binary_cache(tmp_path):

@@ -15,7 +15,15 @@ from conftest import closed_form_graphs, csr_from_edges
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-VARIANTS = ["wave", "lds", "prefetch", "async", "stepwise"]
+# variant families: the sub-round kernel's three forms, the cascade-queue kernel, and the
+# reference-shaped host level loop; the workload tests run once per family
+FAMILIES = {"subround": ["wave", "lds", "prefetch"], "async": ["async"], "stepwise": ["stepwise"]}
+VARIANTS = [v for f in FAMILIES.values() for v in f]
+
+
+@pytest.fixture(params=list(FAMILIES))
+def family(request):
+    return request.param
 
 
 def say(*a):
@@ -23,7 +31,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def peel_all(engine, g, variants=VARIANTS, **kw):
+def peel_all(engine, g, variants, **kw):
     dg = engine.upload(g)
     out = {}
     for v in variants:
@@ -55,17 +63,17 @@ def test_selftest(engine):
 
 
 @pytest.mark.parametrize("case", closed_form_graphs(), ids=lambda c: c[0])
-def test_closed_forms(engine, case):
+def test_closed_forms(engine, case, family):
     name, n, edges, expect = case
-    say("closed form", name)
+    say("closed form", name, family)
     off, nbr = csr_from_edges(n, edges)
     g = K.Graph.from_csr(off, nbr)
     expect = np.array(expect, np.uint32)
-    check_against(g, peel_all(engine, g), expect, int(expect.max()))
+    check_against(g, peel_all(engine, g, FAMILIES[family]), expect, int(expect.max()))
 
 
 @pytest.mark.parametrize("seed", range(24))
-def test_random_small_vs_bruteforce(engine, seed):
+def test_random_small_vs_bruteforce(engine, seed, family):
     rng = np.random.default_rng(100 + seed)
     n = 5 + seed % 12
     p = [0.2, 0.4, 0.6][seed % 3]
@@ -73,25 +81,27 @@ def test_random_small_vs_bruteforce(engine, seed):
     off, nbr = csr_from_edges(n, edges)
     g = K.Graph.from_csr(off, nbr)
     bf, kb = O.bruteforce(off, nbr)
-    check_against(g, peel_all(engine, g), bf, kb)
+    check_against(g, peel_all(engine, g, FAMILIES[family]), bf, kb)
 
 
-def test_networkx_fixtures(engine):
+def test_networkx_fixtures(engine, family):
     z = np.load(f"{GOLDEN}/nx_cores.npz")
     for i in range(int(z["count"])):
-        say("nx fixture", i)
+        say("nx fixture", i, family)
         g = K.Graph.from_csr(z[f"off{i}"], z[f"nbr{i}"])
         expect = z[f"core{i}"]
-        check_against(g, peel_all(engine, g), expect, int(expect.max()))
+        check_against(g, peel_all(engine, g, FAMILIES[family]), expect, int(expect.max()))
 
 
-def test_er_100k_1m_all_variants(engine):
-    say("ER n=100000 m=1000000 seed 1")
+def test_er_100k_1m_all_variants(engine, family):
+    say("ER n=100000 m=1000000 seed 1", family)
     g = K.Graph.erdos_renyi(100000, 1000000, 1)
     core, kmax = O.bz(g.neighbors_offset, g.neighbors)
     ls = O.levelsync(g.neighbors_offset, g.neighbors)
     assert (ls["core"] == core).all()
-    check_against(g, peel_all(engine, g), core, kmax)
+    check_against(g, peel_all(engine, g, FAMILIES[family]), core, kmax)
+    if family != "subround":
+        return
     # reference level loop (no level skipping) and other workgroup sizes give the same result
     res = peel_all(engine, g, ["wave"], flags=K.FLAG_NO_LEVEL_SKIP)
     check_against(g, res, core, kmax)
@@ -100,11 +110,13 @@ def test_er_100k_1m_all_variants(engine):
         check_against(g, peel_all(engine, g, ["wave"], block=block), core, kmax)
 
 
-def test_chung_lu_400k_variants(engine):
-    say("Chung-Lu n=400000 gamma=2.1 seed 2")
+def test_chung_lu_400k_variants(engine, family):
+    say("Chung-Lu n=400000 gamma=2.1 seed 2", family)
     g = K.Graph.chung_lu(400000, 2)
     core, kmax = O.bz(g.neighbors_offset, g.neighbors)
-    check_against(g, peel_all(engine, g), core, kmax)
+    check_against(g, peel_all(engine, g, FAMILIES[family]), core, kmax)
+    if family != "subround":
+        return
     # hub arcs cut into chunks at the smallest threshold, and never cut
     check_against(g, peel_all(engine, g, ["wave"], heavy_threshold=16, chunk_arcs=64), core, kmax)
     check_against(g, peel_all(engine, g, ["lds"], heavy_threshold=256, chunk_arcs=1000), core, kmax)
@@ -126,7 +138,7 @@ def test_star_and_hub_chunks(engine):
                       core, kmax)
 
 
-def test_degenerate_inputs(engine):
+def test_degenerate_inputs(engine, family):
     for name, (off, nbr) in {
         "one_vertex": (np.array([0, 0], np.uint32), np.zeros(0, np.uint32)),
         "edgeless_1000": (np.zeros(1001, np.uint32), np.zeros(0, np.uint32)),
@@ -136,7 +148,7 @@ def test_degenerate_inputs(engine):
         say("degenerate", name)
         g = K.Graph.from_csr(off, nbr)
         core, kmax = O.bz(off, nbr)
-        check_against(g, peel_all(engine, g), core, kmax)
+        check_against(g, peel_all(engine, g, FAMILIES[family]), core, kmax)
 
 
 def test_repeat_peels_identical(engine):
@@ -146,20 +158,31 @@ def test_repeat_peels_identical(engine):
     for i in range(5):
         r = dg.peel("wave")
         assert r.kmax == kmax and (r.core == core).all(), i
-    # the uploaded degrees are untouched: a peel after a different variant still matches, and
-    # variants alternate freely on one graph (the queue kernel re-clears what others wrote)
-    for v in ("prefetch", "async", "lds", "async", "async", "wave"):
+    # the uploaded degrees are untouched: a peel after a different variant still matches
+    for v in ("prefetch", "lds", "wave"):
         r = dg.peel(v)
         assert (r.core == core).all() and r.kmax == kmax, v
     dg.free()
 
 
-def test_rmat20_levels_and_shells(engine):
-    say("R-MAT scale 20 seed 3")
+def test_repeat_peels_alternating_with_async(engine):
+    """Variants alternate freely on one graph: the cascade-queue kernel re-clears the queue slots
+    and chunk slots that earlier peels wrote."""
+    g = K.Graph.rmat(16, 17)
+    core, kmax = O.bz(g.neighbors_offset, g.neighbors)
+    dg = engine.upload(g)
+    for v in ("prefetch", "async", "lds", "async", "async", "wave", "stepwise", "async"):
+        r = dg.peel(v)
+        assert (r.core == core).all() and r.kmax == kmax, v
+    dg.free()
+
+
+def test_rmat20_levels_and_shells(engine, family):
+    say("R-MAT scale 20 seed 3", family)
     g = K.Graph.rmat(20, 3)
     ls = O.levelsync(g.neighbors_offset, g.neighbors, want_order=True)
     core, kmax = ls["core"], ls["kmax"]
-    res = peel_all(engine, g)
+    res = peel_all(engine, g, FAMILIES[family])
     check_against(g, res, core, kmax)
     for v, (r, order, shells) in res.items():
         assert (shells == ls["shells"]).all(), v  # level boundaries are schedule-independent
@@ -182,6 +205,7 @@ def test_rmat24_full_size(engine):
     dg.free()
 
 
+@pytest.mark.late  # its candidates include the cascade-queue variant
 def test_tuner_picks_valid_params_and_caches(engine, tmp_path):
     g = K.Graph.chung_lu(100000, 31)
     ref, km = O.bz(g.neighbors_offset, g.neighbors)
