@@ -95,8 +95,12 @@ enum {
     KC_FLAG_GRAPH = 1u << 9,         /* kc_batch_peel: launch the whole peel (state reset, degree
                                         copy, persistent kernel) as one HIP graph, captured on first
                                         use for each (block, flags) and replayed afterwards */
-    KC_FLAG_DIRECT_POLL = 1u << 10   /* ASYNC: idle workgroups poll the shared queue counters
+    KC_FLAG_DIRECT_POLL = 1u << 10,  /* ASYNC: idle workgroups poll the shared queue counters
                                         directly instead of their XCD group's work hint */
+    KC_FLAG_LEVEL_HANDOFF = 1u << 11 /* ASYNC: no grid barriers — each workgroup's scan is a work
+                                        item, processing waits only for the level's scan count,
+                                        and the thread that finds a level complete publishes the
+                                        next one. Experimental: never chosen by the tuner */
 };
 
 typedef struct {

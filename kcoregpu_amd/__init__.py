@@ -30,6 +30,7 @@ FLAG_NO_LEVEL_SKIP = 1
 FLAG_TRACE = 1 << 8
 FLAG_GRAPH = 1 << 9
 FLAG_DIRECT_POLL = 1 << 10
+FLAG_LEVEL_HANDOFF = 1 << 11
 
 KCG_ERRORS = {1: "I/O error", 2: "vertex id or arc count beyond uint32", 3: "bad argument",
               4: "malformed serialized cache"}

@@ -150,10 +150,14 @@ static int graph_arrays(kc_graph* g, bool alive, void** ptr[13], size_t bytes[13
 
 static uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
-template <int BLOCK, bool PREFETCH, bool ASYNC>
+// KIND: 0 the sub-round kernel, 1 the cascade-queue kernel, 2 the cascade-queue kernel with the
+// level hand-off (no grid barriers)
+template <int BLOCK, bool PREFETCH, int KIND>
 static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out) {
     int per_cu = 0;
-    if (ASYNC) {
+    if (KIND == 2) {
+        KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_async_nb<BLOCK, PREFETCH>, BLOCK, 0));
+    } else if (KIND == 1) {
         KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_async<BLOCK, PREFETCH>, BLOCK, 0));
     } else {
         KC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peel_persistent<BLOCK, PREFETCH>, BLOCK, 0));
@@ -165,7 +169,9 @@ static kc_status launch_peel(kc_ctx* c, const PeelArgs& args, uint32_t* grid_out
     if (a.lq_depth > (uint32_t)WgState<BLOCK>::LQ) a.lq_depth = WgState<BLOCK>::LQ;
     // A plain launch: the kernel's elastic start (register_workgroup) only lets workgroups
     // that are resident together take part, so it needs no co-residency guarantee.
-    if (ASYNC) {
+    if (KIND == 2) {
+        peel_async_nb<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
+    } else if (KIND == 1) {
         peel_async<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
     } else {
         peel_persistent<BLOCK, PREFETCH><<<dim3(grid), dim3(BLOCK), 0, c->stream>>>(a);
@@ -528,17 +534,23 @@ kc_status kc_peel(kc_ctx* c, kc_graph* g, const kc_params* p, uint32_t* core_out
                                              reinterpret_cast<uint4*>(g->d_core), g->padV / 4);
     KC_TRY(hipGetLastError());
     const bool pf = variant == KC_VARIANT_PREFETCH;
-    if (async) {
+    if (async && (prm.flags & KC_FLAG_LEVEL_HANDOFF)) {
         switch (block) {
-            case 256: s = launch_peel<256, false, true>(c, a, &grid); break;
-            case 512: s = launch_peel<512, false, true>(c, a, &grid); break;
-            default: s = launch_peel<1024, false, true>(c, a, &grid); break;
+            case 256: s = launch_peel<256, false, 2>(c, a, &grid); break;
+            case 512: s = launch_peel<512, false, 2>(c, a, &grid); break;
+            default: s = launch_peel<1024, false, 2>(c, a, &grid); break;
+        }
+    } else if (async) {
+        switch (block) {
+            case 256: s = launch_peel<256, false, 1>(c, a, &grid); break;
+            case 512: s = launch_peel<512, false, 1>(c, a, &grid); break;
+            default: s = launch_peel<1024, false, 1>(c, a, &grid); break;
         }
     } else {
         switch (block) {
-            case 256: s = pf ? launch_peel<256, true, false>(c, a, &grid) : launch_peel<256, false, false>(c, a, &grid); break;
-            case 512: s = pf ? launch_peel<512, true, false>(c, a, &grid) : launch_peel<512, false, false>(c, a, &grid); break;
-            default: s = pf ? launch_peel<1024, true, false>(c, a, &grid) : launch_peel<1024, false, false>(c, a, &grid); break;
+            case 256: s = pf ? launch_peel<256, true, 0>(c, a, &grid) : launch_peel<256, false, 0>(c, a, &grid); break;
+            case 512: s = pf ? launch_peel<512, true, 0>(c, a, &grid) : launch_peel<512, false, 0>(c, a, &grid); break;
+            default: s = pf ? launch_peel<1024, true, 0>(c, a, &grid) : launch_peel<1024, false, 0>(c, a, &grid); break;
         }
     }
     if (s != KC_OK) return s;

@@ -50,6 +50,10 @@ enum : uint32_t {
     A_SVEND,            // statically over every wave before the queues open
     A_SCBEG,
     A_SCEND,
+    A_SDONE,            // level hand-off: scan shares done over the whole peel (npart per level)
+    A_GMIN0,            // level hand-off: smallest degree above L seen in even / odd levels,
+    A_GMIN1,            // stored complemented (0 = none yet)
+    A_LCLAIM,           // level hand-off: generation of the last level whose completion was won
     A_HINT,             // kXcds work hints: advanced after a workgroup appended crossings
     A_LDONE = A_HINT + kXcds,  // kXcds level-done flags
     A_COUNT = A_LDONE + kXcds
@@ -60,6 +64,7 @@ struct WgStateA : WgState<BLOCK> {
     uint32_t surv[BLOCK / 64][kStageCap];
     uint32_t kind, beg, n;
     uint32_t vcount, ccount;
+    uint32_t hkind, hbeg, hn;  // level hand-off: a claim of the next level, held for it (hkind 3: none)
 };
 
 __device__ __forceinline__ void order_point() {
@@ -458,6 +463,294 @@ __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
         mn = kSent;
         __syncthreads();
         if (threadIdx.x == 0) S.mn = kSent;
+        __syncthreads();
+    }
+    if (S.vbid == 0 && threadIdx.x == 0) {
+        uint32_t* r = cw(a, C_RESULT);
+        r[R_ERR] = ld_rlx(cw(a, C_ERR));
+        r[R_KMAX] = L;
+        r[R_LEVELS] = ld_rlx(cw(a, C_LEVELS));
+        r[R_SUBROUNDS] = 0;
+        r[R_NPART] = S.npart;
+        r[R_NTRACE] = ld_rlx(cw(a, C_NTRACE));
+        r[R_WORDS] = ld_rlx(cw(a, A_SCAN_DENSE));
+        r[R_WORDS + 1] = ld_rlx(cw(a, A_SCAN_LIST));
+        r[R_WORDS + 2] = ld_rlx(cw(a, A_SURV));
+        r[R_WORDS + 3] = ld_rlx(cw(a, C_CTAIL));
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// peel_async_nb (KC_FLAG_LEVEL_HANDOFF): the cascade-queue kernel without grid barriers.
+//  * A workgroup's scan share is a work item too: after its appends are complete it adds 1 to
+//    A_SDONE, so a level can only be complete once npart scans of it are counted.
+//  * Every wave folds the smallest degree above L it saw into the level's A_GMIN word (as a
+//    complement, so a zeroed word means "none") before its workgroup counts work done.
+//  * Whoever reads the level complete (scans, entries and chunks done == npart * gen, OTAIL,
+//    CTAIL; done counters first) wins it with a compare-and-swap on A_LCLAIM, computes the next
+//    level, the shell offsets and the next scan's compaction, and publishes them with the level's
+//    generation in the level-done flags; each workgroup starts the next scan as soon as it sees
+//    its group's flag. No workgroup waits for the others between levels.
+// The scan's frontier is not dealt statically: every entry is claimed from the queues.
+//  * A workgroup still in level g may claim entries that a faster one, already in level g + 1,
+//    appended. After every claim it reads A_LCLAIM: if level g has been won, nothing of g is
+//    left, and g + 1 cannot be complete without this workgroup's scan, so the claim is of
+//    level g + 1 — it is processed with that level (published before the flags; the workgroup
+//    waits for its group's flag to read it) and counted for it, and the workgroup leaves level g.
+//    If g has not been won, no flag says g yet, so nothing of g + 1 has been appended.
+
+__device__ __forceinline__ bool level_complete_nb(const PeelArgs& a, uint32_t npart, uint32_t gen) {
+    const uint32_t sd = ld_rlx(cw(a, A_SDONE));
+    const uint32_t vd = ld_rlx(cw(a, A_VDONE));
+    const uint32_t cd = ld_rlx(cw(a, A_CDONE));
+    order_point();  // the done counters are read before the tails
+    const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
+    const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
+    return sd == npart * gen && vd == vt && cd == ct;
+}
+
+// This wave's smallest degree above L into the level's word; mn restarts. All lanes active.
+__device__ __forceinline__ void fold_min(const PeelArgs& a, uint32_t gen, uint32_t& mn) {
+    const uint32_t m = wave_min(mn);
+    if (lane_id() == 0 && m != kSent)
+        __hip_atomic_fetch_max(cw(a, A_GMIN0 + (gen & 1u)), ~m, __ATOMIC_RELAXED, KC_AGENT);
+    mn = kSent;
+}
+
+// Called by one thread that read level `gen` complete: the winner of the level publishes what
+// the next level needs, then the level-done flags.
+__device__ void hand_off_level(const PeelArgs& a, uint32_t L, uint32_t gen) {
+    if (!cas_rlx(cw(a, A_LCLAIM), gen - 1, gen)) return;
+    const uint32_t t = ld_rlx(cw(a, C_OTAIL));
+    const uint32_t gm = ~ld_rlx(cw(a, A_GMIN0 + (gen & 1u)));
+    uint32_t phase = PH_NEXT_LEVEL, next = L + 1;
+    if (!(a.flags & FLAG_NO_LEVEL_SKIP) && gm != kSent && gm > next) next = gm;
+    add_rlx(cw(a, C_LEVELS), 1u);
+    // this level's scan, accounted in the mode it ran in; then the alive list it built adopted
+    const uint32_t mode = ld_rlx(cw(a, A_MODE));
+    if (mode == 0) {
+        add_rlx(cw(a, A_SCAN_DENSE), a.nvec * 4);
+    } else {
+        add_rlx(cw(a, A_SCAN_LIST), ld_rlx(cw(a, A_LEN)));
+    }
+    if (ld_rlx(cw(a, A_COMPACT))) {
+        const uint32_t dst = ld_rlx(cw(a, A_DST));
+        const uint32_t len = ld_rlx(cw(a, A_ALEN0 + dst));
+        add_rlx(cw(a, A_SURV), len);
+        st_rlx(cw(a, A_MODE), 1u);
+        st_rlx(cw(a, A_LIST), dst);
+        st_rlx(cw(a, A_LEN), len);
+        st_rlx(cw(a, A_COMPACT), 0u);
+    }
+    if (t >= a.V) {
+        a.shell[L + 1] = t;
+        phase = PH_DONE;
+        next = L;
+    } else if (next > a.dmax) {
+        __hip_atomic_fetch_max(cw(a, C_ERR), 2u, __ATOMIC_RELAXED, KC_AGENT);
+        phase = PH_DONE;
+    } else {
+        for (uint32_t k = L + 1; k <= next; ++k) a.shell[k] = t;
+        // the next scan compacts when few of the entries it would read are still unpeeled
+        const uint32_t m2 = ld_rlx(cw(a, A_MODE));
+        const uint64_t reads = m2 ? ld_rlx(cw(a, A_LEN)) : (uint64_t)a.nvec * 4;
+        const uint64_t unpeeled = a.V - t;
+        if (a.compact_ratio && unpeeled * a.compact_ratio < reads) {
+            const uint32_t dst = m2 ? ld_rlx(cw(a, A_LIST)) ^ 1u : 0u;
+            st_rlx(cw(a, A_ALEN0 + dst), 0u);
+            st_rlx(cw(a, A_DST), dst);
+            st_rlx(cw(a, A_COMPACT), 1u);
+        }
+    }
+    st_rlx(cw(a, A_GMIN0 + ((gen + 1) & 1u)), 0u);  // the next level's word starts empty
+    st_rlx(cw(a, A_PHASE), phase);
+    st_rlx(cw(a, A_NEXT), next);
+    trace_rec(a, TR_LEVEL, 0, L);
+    drain_memory();  // everything above is complete before the flags say so
+    signal_level_done(a, gen);
+}
+
+// Gate, then claim, process, append, fold, count — until the level-done flag carries `gen`.
+template <int BLOCK, bool PREFETCH>
+__device__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen) {
+    constexpr uint32_t WPB = BLOCK / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wid = threadIdx.x / 64;
+    uint32_t* ldone = cw(a, A_LDONE + S.vbid % kXcds);
+    uint32_t sn = 0, mn = kSent;
+    if (threadIdx.x == 0) {
+        // Nothing of level g is processed before every scan of g is counted: a scan must see
+        // the degrees as the previous level left them. A decrement that crosses L + 1 -> L
+        // would otherwise be selected again by a later scan, and the undo of a decrement below
+        // L would hide a vertex of degree L from it.
+        Waiter w;
+        uint32_t nap = 0;
+        while (ld_rlx(cw(a, A_SDONE)) < S.npart * gen) {
+            if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+            idle_sleep(++nap, a.poll_max);
+        }
+        order_point();
+        if (ld_rlx(cw(a, C_ERR))) S.hkind = 3;
+    }
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t kind = 3, beg = 0, n = 0, nap = 0;
+            if (S.hkind != 3) {
+                // the claim this workgroup made, and held, while leaving the previous level
+                kind = S.hkind, beg = S.hbeg, n = S.hn;
+                S.hkind = 3;
+            } else {
+                Waiter w;
+                uint32_t* hint = cw(a, A_HINT + S.vbid % kXcds);
+                const bool direct = (a.flags & FLAG_DIRECT_POLL) != 0;
+                uint32_t seen = direct ? 0 : ld_rlx(hint);
+                bool check = true;
+                while (true) {
+                    if (check) {
+                        const uint32_t ct = ld_rlx(cw(a, C_CTAIL));
+                        const uint32_t ch = ld_rlx(cw(a, A_CHEAD));
+                        if (ch < ct) {
+                            const uint32_t k = min(WPB, ct - ch);
+                            if (cas_rlx(cw(a, A_CHEAD), ch, ch + k)) {
+                                kind = 1, beg = ch, n = k;
+                                break;
+                            }
+                            continue;
+                        }
+                        const uint32_t vt = ld_rlx(cw(a, C_OTAIL));
+                        const uint32_t vh = ld_rlx(cw(a, A_VHEAD));
+                        if (vh < vt) {
+                            const uint32_t k = min((uint32_t)BLOCK, vt - vh);
+                            if (cas_rlx(cw(a, A_VHEAD), vh, vh + k)) {
+                                kind = 2, beg = vh, n = k;
+                                break;
+                            }
+                            continue;
+                        }
+                    }
+                    if (ld_rlx(ldone) == gen) break;
+                    if ((++nap & 31) == 0 && level_complete_nb(a, S.npart, gen)) hand_off_level(a, L, gen);
+                    if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                    idle_sleep(nap, a.poll_max);
+                    if (direct) {
+                        check = true;
+                    } else {
+                        // full check when the group's hint moved, and every 64 polls regardless
+                        const uint32_t h = ld_rlx(hint);
+                        check = h != seen || (nap & 63) == 0;
+                        seen = h;
+                    }
+                }
+                if (kind != 3) {
+                    // The claim is complete before the level word is read. A_LCLAIM, not this
+                    // group's flag, decides: the winner sets it before writing any flag, and the
+                    // flags of the XCD groups become visible one by one, so an entry appended in
+                    // g + 1 by a workgroup that saw its own flag can be claimed here while this
+                    // group's flag still reads g - 1.
+                    order_point();
+                    if (ld_rlx(cw(a, A_LCLAIM)) == gen) {
+                        // a claim of g + 1: held until this workgroup has scanned g + 1 and the
+                        // gate of g + 1 is passed; the level ends here once this group's flag,
+                        // written after the next level's parameters, says so
+                        S.hkind = kind, S.hbeg = beg, S.hn = n;
+                        kind = 3;
+                        Waiter w2;
+                        while (ld_rlx(ldone) != gen) {
+                            if (w2.expired(cw(a, C_ERR), a.time_limit)) break;
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                    }
+                }
+            }
+            S.kind = kind;
+            S.beg = beg;
+            S.n = n;
+        }
+        __syncthreads();
+        const uint32_t kind = S.kind, beg = S.beg, n = S.n;
+        if (kind == 3) break;
+        if (kind == 1) {
+            if (wid < n) {
+                uint64_t ch = 0;
+                if (lane == 0) {
+                    Waiter w;
+                    while ((ch = ld_rlx64(a.chunks + beg + wid)) == 0) {
+                        if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                ch = (uint64_t)bcast_first((uint32_t)ch) | ((uint64_t)bcast_first((uint32_t)(ch >> 32)) << 32);
+                if (ch) process_chunk<BLOCK, PREFETCH>(a, S, L, ch, mn, sn, false);
+            }
+        } else {
+            for (uint32_t j0 = wid * 64; j0 < n; j0 += BLOCK) {
+                const bool mine = j0 + lane < n;
+                uint32_t v = kSent;
+                if (mine) {
+                    Waiter w;
+                    while ((v = ld_rlx(a.order + beg + j0 + lane)) == kSent) {
+                        if (w.expired(cw(a, C_ERR), a.time_limit)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                process_vertices<BLOCK, PREFETCH>(a, S, L, mine && v != kSent, v, mn, sn, false);
+            }
+        }
+        // crossings go to order[] (and the chunk list), and the minimum to the level's word,
+        // before this work counts as done
+        flush_stage(a, S, sn, true, false);
+        fold_min(a, gen, mn);
+        drain_memory();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            publish_hint(a, S);
+            add_rlx(cw(a, kind == 1 ? A_CDONE : A_VDONE), n);
+            order_point();
+            if (level_complete_nb(a, S.npart, gen)) hand_off_level(a, L, gen);
+        }
+        // S.kind is rewritten only by thread 0 after this barrier's readers are done
+        __syncthreads();
+    }
+}
+
+template <int BLOCK, bool PREFETCH>
+__global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
+    __shared__ WgStateA<BLOCK> S;
+    if (!register_workgroup<BLOCK>(a, S)) return;
+    if (threadIdx.x == 0) {
+        S.appended = 0;
+        S.hkind = 3;
+    }
+    __syncthreads();
+    uint32_t L = 0, gen = 0;
+    while (true) {
+        ++gen;
+        // published with the previous level's done flag (or zero before the first level)
+        const uint32_t mode = ld_rlx(cw(a, A_MODE));
+        const uint32_t list = ld_rlx(cw(a, A_LIST));
+        const uint32_t len = ld_rlx(cw(a, A_LEN));
+        const bool compact = ld_rlx(cw(a, A_COMPACT)) != 0;
+        const uint32_t dst = ld_rlx(cw(a, A_DST));
+        uint32_t mn = kSent;
+        if (mode == 0) {
+            scan_dense<BLOCK>(a, S, L, mn, compact, dst);
+        } else {
+            scan_list<BLOCK>(a, S, L, mn, a.alive[list], len, compact, dst);
+        }
+        fold_min(a, gen, mn);
+        drain_memory();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            publish_hint(a, S);
+            add_rlx(cw(a, A_SDONE), 1u);
+            order_point();
+            if (level_complete_nb(a, S.npart, gen)) hand_off_level(a, L, gen);
+        }
+        __syncthreads();
+        process_nb<BLOCK, PREFETCH>(a, S, L, gen);
+        if (ld_rlx(cw(a, C_ERR))) break;
+        if (ld_rlx(cw(a, A_PHASE)) == PH_DONE) break;
+        L = ld_rlx(cw(a, A_NEXT));
         __syncthreads();
     }
     if (S.vbid == 0 && threadIdx.x == 0) {

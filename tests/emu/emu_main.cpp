@@ -3,7 +3,8 @@
 // with the oracle). The kernels are the unchanged device headers of kcoregpu_amd/csrc; the
 // argument set-up mirrors what kcore_hip.hip and serve.hip do on the GPU.
 //
-//   emu peel <variant: persistent|prefetch|async> <graph> <out> [heavy chunk flags grid ratio]
+//   emu peel <variant: persistent|prefetch|async|asyncnb> <graph> <out> [heavy chunk flags grid ratio]
+//            (asyncnb: the cascade-queue kernel with the level hand-off, no grid barriers)
 //   emu stepwise <graph> <out> [flags]  (the host level loop of kcore_hip.hip's peel_stepwise)
 //   emu batch <graphs...> <out>
 //   emu serve <graphs...> <out>          (every request published before the kernel starts)
@@ -76,7 +77,7 @@ int run_peel(int argc, char** argv) {
     uint32_t* core = zalloc<uint32_t>(padV);
     for (uint32_t v = 0; v < padV; ++v) deg0[v] = v < V ? g.off[v + 1] - g.off[v] : kSent;
     std::memcpy(core, deg0, (size_t)padV * 4);
-    const bool async = variant == "async";
+    const bool async = variant == "async" || variant == "asyncnb";
     uint32_t* order = zalloc<uint32_t>(padV);
     if (async)
         for (uint32_t v = 0; v < padV; ++v) order[v] = kSent;
@@ -112,7 +113,9 @@ int run_peel(int argc, char** argv) {
     a.async_mode = async ? 1 : 0;
     a.compact_ratio = ratio;
     a.poll_max = 2;
-    if (variant == "async") {
+    if (variant == "asyncnb") {
+        kcemu::launch(grid, 256, [&] { peel_async_nb<256, false>(a); });
+    } else if (variant == "async") {
         kcemu::launch(grid, 256, [&] { peel_async<256, false>(a); });
     } else if (variant == "prefetch") {
         kcemu::launch(grid, 256, [&] { peel_persistent<256, true>(a); });

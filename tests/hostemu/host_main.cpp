@@ -86,6 +86,8 @@ int run_peel(char** argv) {
         {KC_VARIANT_ASYNC, 256, 16, 64, 0, 1},               // compaction at every chance
         {KC_VARIANT_ASYNC, 0, 0, 0, 0, 0},                   // the same graph again: queues re-armed
         {KC_VARIANT_ASYNC, 512, 0, 0, KC_FLAG_DIRECT_POLL | KC_FLAG_NO_LEVEL_SKIP, 0},
+        {KC_VARIANT_ASYNC, 256, 16, 64, KC_FLAG_LEVEL_HANDOFF, 1},  // level hand-off, no grid barriers
+        {KC_VARIANT_ASYNC, 0, 0, 0, KC_FLAG_LEVEL_HANDOFF | KC_FLAG_NO_LEVEL_SKIP, 0},
         {0, 0, 0, 0, 0, 0},                                  // defaults after the cascade-queue runs
     };
     for (const Run& r : runs) {

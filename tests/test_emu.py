@@ -93,7 +93,7 @@ def graphs():
 GRAPHS = graphs()
 
 
-@pytest.mark.parametrize("variant", ["persistent", "async"])
+@pytest.mark.parametrize("variant", ["persistent", "async", "asyncnb"])
 @pytest.mark.parametrize("case", GRAPHS, ids=[c[0] for c in GRAPHS])
 def test_single_graph_kernels(emu, tmp_path, variant, case):
     name, off, nbr = case
@@ -107,10 +107,11 @@ def test_single_graph_kernels(emu, tmp_path, variant, case):
     dict(heavy=64, chunk=512, flags=0, grid=5, ratio=64),
     dict(heavy=0xFFFFFFFF, chunk=512, flags=1024, grid=4, ratio=4),  # never chunk; direct polling
 ])
-def test_async_settings(emu, tmp_path, settings):
+@pytest.mark.parametrize("variant", ["async", "asyncnb"])
+def test_async_settings(emu, tmp_path, settings, variant):
     g = K.Graph.rmat(13, 9)
     off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
-    res = peel(emu, tmp_path, off, nbr, "async", settings["heavy"], settings["chunk"], settings["flags"],
+    res = peel(emu, tmp_path, off, nbr, variant, settings["heavy"], settings["chunk"], settings["flags"],
                settings["grid"], settings["ratio"])
     check(off, nbr, *res[:5])
     info = dict(zip(res[5].split()[0::2], map(int, res[5].split()[1::2])))
@@ -202,7 +203,7 @@ def test_serve_lease(emu):
 WEAK_CASES = [c for c in GRAPHS if c[0] in ("K12", "er5000", "rmat12", "hub3000", "chung_lu8000", "union")]
 
 
-@pytest.mark.parametrize("variant", ["persistent", "async", "prefetch"])
+@pytest.mark.parametrize("variant", ["persistent", "async", "asyncnb", "prefetch"])
 @pytest.mark.parametrize("case", WEAK_CASES, ids=[c[0] for c in WEAK_CASES])
 def test_weak_memory_single_graph(emu, tmp_path, variant, case):
     name, off, nbr = case
@@ -213,10 +214,11 @@ def test_weak_memory_single_graph(emu, tmp_path, variant, case):
     dict(heavy=16, chunk=64, flags=0, grid=3, ratio=1),
     dict(heavy=64, chunk=512, flags=1024, grid=5, ratio=4),
 ])
-def test_weak_memory_async_settings(emu, tmp_path, settings):
+@pytest.mark.parametrize("variant", ["async", "asyncnb"])
+def test_weak_memory_async_settings(emu, tmp_path, settings, variant):
     g = K.Graph.rmat(13, 9)
     off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
-    res = peel(emu, tmp_path, off, nbr, "async", settings["heavy"], settings["chunk"], settings["flags"],
+    res = peel(emu, tmp_path, off, nbr, variant, settings["heavy"], settings["chunk"], settings["flags"],
                settings["grid"], settings["ratio"], env=WEAK)
     check(off, nbr, *res[:5])
 
@@ -283,7 +285,7 @@ def test_weak_memory_catches_a_missing_drain(tmp_path):
     assert mismatches(WEAK) != 0
 
 
-@pytest.mark.parametrize("variant", ["persistent", "async"])
+@pytest.mark.parametrize("variant", ["persistent", "async", "asyncnb"])
 def test_weak_memory_with_jitter(emu, tmp_path, variant):
     """Weak memory plus random pauses of whole workgroups (KCEMU_JITTER): workgroups overtake each
     other at random points of the claim, termination and barrier protocols."""

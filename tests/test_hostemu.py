@@ -104,7 +104,8 @@ def graph_set():
 
 GRAPHS = graph_set()
 RUNS = ["defaults", "wave 256 chunks", "prefetch 1024", "lds no-skip", "stepwise", "stepwise no-skip",
-        "async ratio 1", "async again", "async direct no-skip", "defaults again"]
+        "async ratio 1", "async again", "async direct no-skip", "handoff ratio 1", "handoff no-skip",
+        "defaults again"]
 
 
 @pytest.mark.parametrize("case", GRAPHS, ids=[c[0] for c in GRAPHS])
