@@ -118,3 +118,25 @@ def test_ctypes_signatures_match_the_headers(header, which):
         elif ret in ("double", "uint64_t", "float"):
             assert fn.restype is _SCALAR[ret], (name, ret, fn.restype)
     assert checked >= len(protos) // 2, (checked, len(protos))
+
+
+def test_python_constants_match_the_header_enums(tmp_path):
+    """The Python names of the variants and flags have the values kcore.h gives them, read back from
+    the C compiler, and every KC_FLAG_ / KC_VARIANT_ of the header has a Python name."""
+    import kcoregpu_amd as K
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "kcore.h")).read(), flags=re.S)
+    flags = sorted(set(re.findall(r"\bKC_FLAG_([A-Z_]+)\s*=", text)))
+    variants = sorted(set(re.findall(r"\bKC_VARIANT_([A-Z_]+)\s*=", text)))
+    assert len(flags) >= 5 and len(variants) >= 6
+    lines = ['#include <stdio.h>', '#include "kcore.h"', 'int main(void) {']
+    lines += [f'    printf("FLAG_{n} %u\\n", (unsigned)KC_FLAG_{n});' for n in flags]
+    lines += [f'    printf("VARIANT_{n} %u\\n", (unsigned)KC_VARIANT_{n});' for n in variants]
+    lines += ['    return 0;', '}']
+    src, exe = tmp_path / "enums.c", tmp_path / "enums"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for n in flags:
+        assert getattr(K, f"FLAG_{n}") == int(got[f"FLAG_{n}"]), n
+    assert {k.upper(): v for k, v in K.VARIANTS.items()} == {n: int(got[f"VARIANT_{n}"]) for n in variants}
