@@ -54,6 +54,7 @@ enum : uint32_t {
     A_GMIN0,            // level hand-off: smallest degree above L seen in even / odd levels,
     A_GMIN1,            // stored complemented (0 = none yet)
     A_LCLAIM,           // level hand-off: generation of the last level whose completion was won
+    A_HELD,             // level hand-off: claims of the next level held for it (reported only)
     A_HINT,             // kXcds work hints: advanced after a workgroup appended crossings
     A_LDONE = A_HINT + kXcds,  // kXcds level-done flags
     A_COUNT = A_LDONE + kXcds
@@ -653,6 +654,7 @@ __device__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, ui
                         // gate of g + 1 is passed; the level ends here once this group's flag,
                         // written after the next level's parameters, says so
                         S.hkind = kind, S.hbeg = beg, S.hn = n;
+                        add_rlx(cw(a, A_HELD), 1u);
                         kind = 3;
                         Waiter w2;
                         while (ld_rlx(ldone) != gen) {
@@ -765,6 +767,7 @@ __global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
         r[R_WORDS + 1] = ld_rlx(cw(a, A_SCAN_LIST));
         r[R_WORDS + 2] = ld_rlx(cw(a, A_SURV));
         r[R_WORDS + 3] = ld_rlx(cw(a, C_CTAIL));
+        r[R_WORDS + 4] = ld_rlx(cw(a, A_HELD));
     }
 }
 

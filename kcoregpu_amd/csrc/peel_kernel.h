@@ -67,6 +67,9 @@ enum : uint32_t {
     C_SNAP = C_REL + kXcds,  // two snapshot slots
     C_COUNT = C_SNAP + 2
 };
+// Result words. From R_WORDS on: dense entries scanned, list entries scanned and alive-list
+// entries written (cascade-queue kernels), chunks queued (every kernel), and claims of the next
+// level held for it (level hand-off only, not read by the host).
 enum : uint32_t { R_ERR = 0, R_KMAX, R_LEVELS, R_SUBROUNDS, R_NPART, R_NTRACE, R_WORDS };
 
 enum : uint32_t { PH_SUBROUND = 1, PH_NEXT_LEVEL = 2, PH_DONE = 3 };

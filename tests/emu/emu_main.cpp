@@ -130,8 +130,9 @@ int run_peel(int argc, char** argv) {
     std::fwrite(order, 4, V, f);
     std::fwrite(a.shell, 4, r[R_KMAX] + 2, f);
     std::fclose(f);
-    std::printf("levels %u subrounds %u participants %u dense %u list %u surv %u chunks %u\n", r[R_LEVELS],
-                r[R_SUBROUNDS], r[R_NPART], r[R_WORDS], r[R_WORDS + 1], r[R_WORDS + 2], r[R_WORDS + 3]);
+    std::printf("levels %u subrounds %u participants %u dense %u list %u surv %u chunks %u held %u\n", r[R_LEVELS],
+                r[R_SUBROUNDS], r[R_NPART], r[R_WORDS], r[R_WORDS + 1], r[R_WORDS + 2], r[R_WORDS + 3],
+                r[R_WORDS + 4]);
     return 0;
 }
 

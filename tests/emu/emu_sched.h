@@ -11,8 +11,9 @@
 // In weak-memory mode (KCEMU_WEAK=1) each work-item has a store buffer (see hip/hip_runtime.h):
 // at every yield each buffered store completes with probability 1/3 (in random order); a sleep,
 // the end of the work-item and a pass in which nothing could run complete them all.
-// KCEMU_JITTER=N: after a scheduling pass, a workgroup pauses for up to 200 us with probability
-// 1/N, so workgroups overtake each other at random points (more cross-workgroup interleavings).
+// KCEMU_JITTER=N[:US]: after a scheduling pass, a workgroup pauses for up to US microseconds
+// (default 200) with probability 1/N, so workgroups overtake each other at random points (more
+// cross-workgroup interleavings); pauses of milliseconds stand for a preempted workgroup.
 // Deadlocks inside a workgroup (every live fiber blocked on something that cannot complete)
 // abort with a message; waits on other workgroups are the kernels' own, bounded by their wall
 // clock limits (clock_ticks() counts 10 ns ticks).
@@ -23,6 +24,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <thread>
 #include <vector>
@@ -66,6 +68,12 @@ inline const bool g_weak = [] {
 inline const uint32_t g_jitter = [] {
     const char* e = std::getenv("KCEMU_JITTER");
     return e && *e ? (uint32_t)std::strtoul(e, nullptr, 10) : 0u;
+}();
+inline const uint32_t g_jitter_us = [] {
+    const char* e = std::getenv("KCEMU_JITTER");
+    const char* c = e ? std::strchr(e, ':') : nullptr;
+    const uint32_t us = c ? (uint32_t)std::strtoul(c + 1, nullptr, 10) : 0u;
+    return us ? us : 200u;
 }();
 
 inline void sb_complete(const SbEntry& e) {
@@ -352,7 +360,7 @@ inline void run_group(Group& g) {
             r ^= r << 13;
             r ^= r >> 7;
             r ^= r << 17;
-            if (r % g_jitter == 0) std::this_thread::sleep_for(std::chrono::microseconds((r >> 20) % 200));
+            if (r % g_jitter == 0) std::this_thread::sleep_for(std::chrono::microseconds((r >> 20) % g_jitter_us));
         }
         (void)any_live;
     }

@@ -293,3 +293,21 @@ def test_weak_memory_with_jitter(emu, tmp_path, variant):
     for name in ("rmat12", "hub3000", "chung_lu8000"):
         _, off, nbr = next(c for c in GRAPHS if c[0] == name)
         check(off, nbr, *peel(emu, tmp_path, off, nbr, variant, 16, 64, 0, 5, 2, env=env)[:5])
+
+
+def test_handoff_held_claims(emu, tmp_path):
+    """The level hand-off's rarest path: a workgroup that claims after the level was won holds
+    entries of the next level, scans that level, and processes them after its gate. Small graphs
+    with many levels, six workgroups and pauses of up to 5 ms (a preempted workgroup) make it
+    happen; the kernel counts the holds (result word R_WORDS + 4) and every run must still be
+    exact."""
+    env = dict(os.environ, KCEMU_JITTER="100:5000")
+    held = 0
+    for seed in range(6):
+        g = K.Graph.rmat(10, 40 + seed, edge_factor=12)
+        off, nbr = g.neighbors_offset.copy(), g.neighbors.copy()
+        res = peel(emu, tmp_path, off, nbr, "asyncnb", 16, 64, 0, 6, 4, env=env)
+        check(off, nbr, *res[:5])
+        info = dict(zip(res[5].split()[0::2], map(int, res[5].split()[1::2])))
+        held += info["held"]
+    assert held > 0
