@@ -42,7 +42,6 @@ enum : uint32_t {
     A_ALEN1,
     A_NEXT,             // next level
     A_PHASE,            // PH_NEXT_LEVEL / PH_DONE
-    A_GEN,              // generation of the current processing phase
     A_SCAN_DENSE,       // scan accounting: dense entries read
     A_SCAN_LIST,        // list entries read (id + gathered degree)
     A_SURV,             // alive-list entries written
