@@ -351,8 +351,7 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
 }
 
 // after the scan: account the scan, adopt a freshly built alive list; an empty level is done
-template <int BLOCK>
-__device__ void hook_scan_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen) {
+__device__ void hook_scan_async(const PeelArgs& a, uint32_t L, uint32_t gen) {
     if (threadIdx.x != 0) return;
     const uint32_t mode = ld_rlx(cw(a, A_MODE));
     if (mode == 0) {
@@ -447,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
             scan_list<BLOCK>(a, S, L, mn, a.alive[list], len, compact, dst);
         }
         ++gen;
-        grid_barrier<BLOCK>(a, S, [&] { hook_scan_async<BLOCK>(a, S, L, gen); });
+        grid_barrier<BLOCK>(a, S, [&] { hook_scan_async(a, L, gen); });
         if (ld_rlx(cw(a, C_ERR))) break;
         if (threadIdx.x == 0) st_rlx(a.dbg + S.vbid * 8 + 2, 2u);
         process_async<BLOCK, PREFETCH>(a, S, L, gen, mn);
