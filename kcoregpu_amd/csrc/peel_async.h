@@ -56,7 +56,8 @@ enum : uint32_t {
     A_HELD,             // level hand-off: claims of the next level held for it (reported only)
     A_HINT,             // kXcds work hints: advanced after a workgroup appended crossings
     A_LDONE = A_HINT + kXcds,  // kXcds level-done flags
-    A_COUNT = A_LDONE + kXcds
+    A_GATE = A_LDONE + kXcds,  // level hand-off: kXcds flags, the generation whose scans are all counted
+    A_COUNT = A_GATE + kXcds
 };
 
 template <int BLOCK>
@@ -482,20 +483,23 @@ __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
 // ---------------------------------------------------------------------------------------
 // peel_async_nb (KC_FLAG_LEVEL_HANDOFF): the cascade-queue kernel without grid barriers.
 //  * A workgroup's scan share is a work item too: after its appends are complete it adds 1 to
-//    A_SDONE, so a level can only be complete once npart scans of it are counted.
+//    A_SDONE, so a level can only be complete once npart scans of it are counted. The scan that
+//    brings the count to npart * gen raises the level's gate flags (one per XCD group).
+//  * Gate: no entry of a level is processed before its gate is open, so no scan of a level ever
+//    overlaps that level's decrements. (A crossing L + 1 -> L would otherwise be selected again
+//    by a later scan, and the undo of a decrement below L would hide a vertex of degree L.)
 //  * Every wave folds the smallest degree above L it saw into the level's A_GMIN word (as a
 //    complement, so a zeroed word means "none") before its workgroup counts work done.
 //  * Whoever reads the level complete (scans, entries and chunks done == npart * gen, OTAIL,
 //    CTAIL; done counters first) wins it with a compare-and-swap on A_LCLAIM, computes the next
 //    level, the shell offsets and the next scan's compaction, and publishes them with the level's
 //    generation in the level-done flags; each workgroup starts the next scan as soon as it sees
-//    its group's flag. No workgroup waits for the others between levels.
-// The scan's frontier is not dealt statically: every entry is claimed from the queues.
-//  * A workgroup still in level g may claim entries that a faster one, already in level g + 1,
+//    its group's flag. Nobody waits for the slowest workgroup to reach a barrier.
+// Every entry, the scan's frontier included, is claimed from the queues.
+//  * A workgroup still in level g may claim entries that a faster one, already scanning g + 1,
 //    appended. After every claim it reads A_LCLAIM: if level g has been won, nothing of g is
-//    left, and g + 1 cannot be complete without this workgroup's scan, so the claim is of
-//    level g + 1 — it is processed with that level (published before the flags; the workgroup
-//    waits for its group's flag to read it) and counted for it, and the workgroup leaves level g.
+//    left, so the claim is of g + 1. It is held: the workgroup leaves level g once its group's
+//    flag says so, scans g + 1, passes that level's gate and processes the held claim first.
 //    If g has not been won, no flag says g yet, so nothing of g + 1 has been appended.
 
 __device__ __forceinline__ bool level_complete_nb(const PeelArgs& a, uint32_t npart, uint32_t gen) {
@@ -584,7 +588,9 @@ __device__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, ui
         // L would hide a vertex of degree L from it.
         Waiter w;
         uint32_t nap = 0;
-        while (ld_rlx(cw(a, A_SDONE)) < S.npart * gen) {
+        // the last scanner of the level raises one flag per XCD group, so the pollers spread
+        uint32_t* gate = cw(a, A_GATE + S.vbid % kXcds);
+        while (ld_rlx(gate) != gen) {
             if (w.expired(cw(a, C_ERR), a.time_limit)) break;
             idle_sleep(++nap, a.poll_max);
         }
@@ -742,7 +748,9 @@ __global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
         __syncthreads();
         if (threadIdx.x == 0) {
             publish_hint(a, S);
-            add_rlx(cw(a, A_SDONE), 1u);
+            // the scan that completes the level's count opens the gate of every XCD group
+            if (add_rlx(cw(a, A_SDONE), 1u) + 1 == S.npart * gen)
+                for (uint32_t x = 0; x < kXcds; ++x) st_rlx(cw(a, A_GATE + x), gen);
             order_point();
             if (level_complete_nb(a, S.npart, gen)) hand_off_level(a, L, gen);
         }
