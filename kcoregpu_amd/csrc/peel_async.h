@@ -78,6 +78,10 @@ __device__ __forceinline__ bool cas_rlx(uint32_t* p, uint32_t expect, uint32_t w
     return __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, KC_AGENT);
 }
 
+// Alive list i (0/1). A select of the two fields, not an index: indexing the argument struct at
+// run time would keep the whole struct in scratch memory.
+__device__ __forceinline__ uint32_t* alive_list(const PeelArgs& a, uint32_t i) { return i ? a.alive[1] : a.alive[0]; }
+
 // Survivors (degree > L) staged per wave and appended to the alive list being built.
 template <int BLOCK>
 __device__ void flush_surv(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t& n, uint32_t dst) {
@@ -85,7 +89,7 @@ __device__ void flush_surv(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t& n, u
     const uint32_t lane = lane_id();
     const uint32_t* st = S.surv[threadIdx.x / 64];
     const uint32_t b = wave_reserve(cw(a, A_ALEN0 + dst), n);
-    for (uint32_t i = lane; i < n; i += 64) st_rlx(a.alive[dst] + b + i, st[i]);
+    for (uint32_t i = lane; i < n; i += 64) st_rlx(alive_list(a, dst) + b + i, st[i]);
     n = 0;
     __builtin_amdgcn_wave_barrier();
 }
@@ -104,7 +108,7 @@ __device__ __forceinline__ void surv_push(const PeelArgs& a, WgStateA<BLOCK>& S,
 
 // Scan of the dense degree array (16-byte SC1 loads, four in flight per lane).
 template <int BLOCK>
-__device__ void scan_dense(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, bool compact,
+__device__ __forceinline__ void scan_dense(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, bool compact,
                            uint32_t dst) {
     constexpr uint32_t WPB = BLOCK / 64;
     constexpr uint32_t U = 4;
@@ -139,7 +143,7 @@ __device__ void scan_dense(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, ui
 
 // Scan of an alive list: ids are read coalesced, their degrees gathered.
 template <int BLOCK>
-__device__ void scan_list(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, const uint32_t* list,
+__device__ __forceinline__ void scan_list(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t& mn, const uint32_t* list,
                           uint32_t len, bool compact, uint32_t dst) {
     constexpr uint32_t WPB = BLOCK / 64;
     constexpr uint32_t U = 4;
@@ -208,7 +212,7 @@ __device__ __forceinline__ void idle_sleep(uint32_t nap, uint32_t poll_max) {
 // peel_persistent's first sub-round); then claim, process, append, count, until the level's done
 // flag carries generation `gen`.
 template <int BLOCK, bool PREFETCH>
-__device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen, uint32_t& mn) {
+__device__ __forceinline__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen, uint32_t& mn) {
     constexpr uint32_t WPB = BLOCK / 64;
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / 64;
@@ -352,7 +356,7 @@ __device__ void process_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L,
 }
 
 // after the scan: account the scan, adopt a freshly built alive list; an empty level is done
-__device__ void hook_scan_async(const PeelArgs& a, uint32_t L, uint32_t gen) {
+__device__ __forceinline__ void hook_scan_async(const PeelArgs& a, uint32_t L, uint32_t gen) {
     if (threadIdx.x != 0) return;
     const uint32_t mode = ld_rlx(cw(a, A_MODE));
     if (mode == 0) {
@@ -384,7 +388,7 @@ __device__ void hook_scan_async(const PeelArgs& a, uint32_t L, uint32_t gen) {
 
 // level end: next level, shell offsets, compaction of the next scan
 template <int BLOCK>
-__device__ void hook_level_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L) {
+__device__ __forceinline__ void hook_level_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L) {
     uint32_t m = kSent;
     for (uint32_t i = threadIdx.x; i < S.npart; i += BLOCK) m = min(m, ld_rlx(a.wgmin + i));
     m = wave_min(m);
@@ -424,7 +428,9 @@ __device__ void hook_level_async(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t
 template <int BLOCK, bool PREFETCH>
 __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
     __shared__ WgStateA<BLOCK> S;
-    if (!register_workgroup<BLOCK>(a, S)) return;
+    bool registered;
+    [[clang::always_inline]] registered = register_workgroup<BLOCK>(a, S);
+    if (!registered) return;
     uint32_t L = 0, gen = 0;
     uint32_t mn = kSent;
     if (threadIdx.x == 0) S.mn = kSent;
@@ -444,10 +450,10 @@ __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
         if (mode == 0) {
             scan_dense<BLOCK>(a, S, L, mn, compact, dst);
         } else {
-            scan_list<BLOCK>(a, S, L, mn, a.alive[list], len, compact, dst);
+            scan_list<BLOCK>(a, S, L, mn, alive_list(a, list), len, compact, dst);
         }
         ++gen;
-        grid_barrier<BLOCK>(a, S, [&] { hook_scan_async(a, L, gen); });
+        [[clang::always_inline]] grid_barrier<BLOCK>(a, S, [&] { hook_scan_async(a, L, gen); });
         if (ld_rlx(cw(a, C_ERR))) break;
         if (threadIdx.x == 0) st_rlx(a.dbg + S.vbid * 8 + 2, 2u);
         process_async<BLOCK, PREFETCH>(a, S, L, gen, mn);
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
         if (lane_id() == 0) __hip_atomic_fetch_min(&S.mn, mn, __ATOMIC_RELAXED, KC_WG);
         __syncthreads();
         if (threadIdx.x == 0) st_rlx(a.wgmin + S.vbid, S.mn);
-        grid_barrier<BLOCK>(a, S, [&] { hook_level_async<BLOCK>(a, S, L); });
+        [[clang::always_inline]] grid_barrier<BLOCK>(a, S, [&] { hook_level_async<BLOCK>(a, S, L); });
         if (ld_rlx(cw(a, C_ERR))) break;
         if (ld_rlx(cw(a, A_PHASE)) == PH_DONE) break;
         L = ld_rlx(cw(a, A_NEXT));
@@ -522,7 +528,7 @@ __device__ __forceinline__ void fold_min(const PeelArgs& a, uint32_t gen, uint32
 
 // Called by one thread that read level `gen` complete: the winner of the level publishes what
 // the next level needs, then the level-done flags.
-__device__ void hand_off_level(const PeelArgs& a, uint32_t L, uint32_t gen) {
+__device__ __forceinline__ void hand_off_level(const PeelArgs& a, uint32_t L, uint32_t gen) {
     if (!cas_rlx(cw(a, A_LCLAIM), gen - 1, gen)) return;
     const uint32_t t = ld_rlx(cw(a, C_OTAIL));
     const uint32_t gm = ~ld_rlx(cw(a, A_GMIN0 + (gen & 1u)));
@@ -575,7 +581,7 @@ __device__ void hand_off_level(const PeelArgs& a, uint32_t L, uint32_t gen) {
 
 // Gate, then claim, process, append, fold, count — until the level-done flag carries `gen`.
 template <int BLOCK, bool PREFETCH>
-__device__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen) {
+__device__ __forceinline__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, uint32_t gen) {
     constexpr uint32_t WPB = BLOCK / 64;
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / 64;
@@ -722,7 +728,9 @@ __device__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S, uint32_t L, ui
 template <int BLOCK, bool PREFETCH>
 __global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
     __shared__ WgStateA<BLOCK> S;
-    if (!register_workgroup<BLOCK>(a, S)) return;
+    bool registered;
+    [[clang::always_inline]] registered = register_workgroup<BLOCK>(a, S);
+    if (!registered) return;
     if (threadIdx.x == 0) {
         S.appended = 0;
         S.hkind = 3;
@@ -741,7 +749,7 @@ __global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
         if (mode == 0) {
             scan_dense<BLOCK>(a, S, L, mn, compact, dst);
         } else {
-            scan_list<BLOCK>(a, S, L, mn, a.alive[list], len, compact, dst);
+            scan_list<BLOCK>(a, S, L, mn, alive_list(a, list), len, compact, dst);
         }
         fold_min(a, gen, mn);
         drain_memory();
