@@ -428,7 +428,7 @@ __device__ __forceinline__ void hook_level_async(const PeelArgs& a, WgStateA<BLO
 template <int BLOCK, bool PREFETCH>
 __global__ __launch_bounds__(BLOCK) void peel_async(PeelArgs a) {
     __shared__ WgStateA<BLOCK> S;
-    bool registered;
+    bool registered;  // calls inlined at their call sites, as in peel_persistent
     [[clang::always_inline]] registered = register_workgroup<BLOCK>(a, S);
     if (!registered) return;
     uint32_t L = 0, gen = 0;
@@ -728,7 +728,7 @@ __device__ __forceinline__ void process_nb(const PeelArgs& a, WgStateA<BLOCK>& S
 template <int BLOCK, bool PREFETCH>
 __global__ __launch_bounds__(BLOCK) void peel_async_nb(PeelArgs a) {
     __shared__ WgStateA<BLOCK> S;
-    bool registered;
+    bool registered;  // calls inlined at their call sites, as in peel_persistent
     [[clang::always_inline]] registered = register_workgroup<BLOCK>(a, S);
     if (!registered) return;
     if (threadIdx.x == 0) {

@@ -558,7 +558,7 @@ __device__ void process_subround(const PeelArgs& a, WgState<BLOCK>& S, uint32_t 
 // Barrier hooks: run by every thread of the last workgroup to arrive; only thread 0 writes.
 
 template <int BLOCK>
-__device__ void level_end(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
+__device__ __forceinline__ void level_end(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
     // smallest degree above L left anywhere: reduce the per-workgroup minima
     uint32_t m = kSent;
     for (uint32_t i = threadIdx.x; i < S.npart; i += BLOCK) m = min(m, ld_rlx(a.wgmin + i));
@@ -589,7 +589,7 @@ __device__ void level_end(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint
 
 // after the select of level L
 template <int BLOCK>
-__device__ void hook_after_scan(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
+__device__ __forceinline__ void hook_after_scan(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
     uint32_t vbeg = 0, vend = 0, cbeg = 0, cend = 0;
     if (threadIdx.x == 0) {
         vbeg = ld_rlx(cw(a, C_LSTART));
@@ -617,7 +617,7 @@ __device__ void hook_after_scan(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L
 }
 
 template <int BLOCK>
-__device__ void hook_after_subround(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
+__device__ __forceinline__ void hook_after_subround(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, uint32_t* snap) {
     uint32_t wbeg = 0, wend = 0, cbeg = 0, cend = 0;
     if (threadIdx.x == 0) {
         wbeg = ld_rlx(cw(a, C_WSTART));
@@ -648,7 +648,11 @@ __device__ void hook_after_subround(const PeelArgs& a, WgState<BLOCK>& S, uint32
 template <int BLOCK, bool PREFETCH>
 __global__ __launch_bounds__(BLOCK) void peel_persistent(PeelArgs a) {
     __shared__ WgState<BLOCK> S;
-    if (!register_workgroup<BLOCK>(a, S)) return;
+    // Every call below is inlined at its call site. An out-of-line callee taking the arguments
+    // by reference would keep them in scratch memory, reloaded from there at every use.
+    bool registered;
+    [[clang::always_inline]] registered = register_workgroup<BLOCK>(a, S);
+    if (!registered) return;
     uint32_t L = 0, nb = 0;
     bool scan = true;
     uint32_t src = 0, vbeg = 0, vend = 0, cbeg = 0, cend = 0;
@@ -663,9 +667,9 @@ __global__ __launch_bounds__(BLOCK) void peel_persistent(PeelArgs a) {
             st_rlx(a.dbg + S.vbid * 8 + 2, scan ? 1u : 2u);
         }
         if (scan) {
-            select_level<BLOCK>(a, S, L, mn);
+            [[clang::always_inline]] select_level<BLOCK>(a, S, L, mn);
         } else {
-            process_subround<BLOCK, PREFETCH>(a, S, L, src, vbeg, vend, cbeg, cend, mn);
+            [[clang::always_inline]] process_subround<BLOCK, PREFETCH>(a, S, L, src, vbeg, vend, cbeg, cend, mn);
         }
         // publish this workgroup's smallest degree above L seen so far in the level
         mn = wave_min(mn);
@@ -673,9 +677,9 @@ __global__ __launch_bounds__(BLOCK) void peel_persistent(PeelArgs a) {
         __syncthreads();
         if (threadIdx.x == 0) st_rlx(a.wgmin + S.vbid, S.mn);
         if (scan)
-            grid_barrier<BLOCK>(a, S, [&] { hook_after_scan<BLOCK>(a, S, L, snap); });
+            [[clang::always_inline]] grid_barrier<BLOCK>(a, S, [&] { hook_after_scan<BLOCK>(a, S, L, snap); });
         else
-            grid_barrier<BLOCK>(a, S, [&] { hook_after_subround<BLOCK>(a, S, L, snap); });
+            [[clang::always_inline]] grid_barrier<BLOCK>(a, S, [&] { hook_after_subround<BLOCK>(a, S, L, snap); });
         nb++;
         if (ld_rlx(cw(a, C_ERR))) break;
         const uint32_t phase = ld_rlx(snap + S_PHASE);
