@@ -41,3 +41,28 @@ def test_drop_in_binding_compiles_and_links(tmp_path):
                         "-L", LIB_DIR, "-lkcore_hip", "-o", str(tmp_path / "libkcore_amd.so")],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_INC, "graph.h")), reason="reference sources absent")
+def test_serving_and_batch_fragments_compile(tmp_path):
+    """The serving loop and the batch example of section 3, wrapped in functions that declare the
+    variables they use (a context, a reference Graph, a request id); the `...` placeholder line is
+    dropped. Every call and struct initialiser must match the headers."""
+    cuda = cuda_h_dir()
+    if cuda is None:
+        pytest.skip("no cuda.h for the reference's common.h")
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    section = text[text.index("## 3."):text.index("## 4.")]
+    blocks = re.findall(r"```cpp\n(.*?)```", section, re.S)
+    assert len(blocks) == 2
+    body = lambda b: "\n".join(ln for ln in b.splitlines() if ln.strip() != "...")
+    code = ("#include <cstdint>\n#include <vector>\n#include \"kcore.h\"\n#include \"kcore_graph.h\"\n"
+            "#include \"graph.h\"\n"
+            "void serving_example(kc_ctx* ctx, Graph& g, uint64_t request_id) {\n" + body(blocks[0]) + "\n}\n"
+            "void batch_example(kc_ctx* ctx) {\n" + body(blocks[1]) + "\n}\n")
+    src = tmp_path / "fragments.cc"
+    src.write_text(code)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wno-unused-result", "-I",
+                        os.path.join(ROOT, "include"), "-I", REF_INC, "-I", cuda, str(src)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
