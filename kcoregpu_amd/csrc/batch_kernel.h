@@ -193,9 +193,14 @@ __device__ uint32_t wg_peel_graph(BatchState<BLOCK>& S, const uint32_t* off, con
                             u[k] = nbr[S.tstart[o] + (j - ex)];
                         }
                     }
+                    // prechecks loaded unconditionally so they overlap (inactive lanes read
+                    // the graph's first offset, read-only), as in the single-graph kernels
                     uint32_t du[kBU], old[kBU];
 #pragma unroll
-                    for (uint32_t k = 0; k < kBU; ++k) du[k] = act[k] ? core[u[k]] : 0;
+                    for (uint32_t k = 0; k < kBU; ++k) {
+                        const uint32_t d = *(act[k] ? core + u[k] : off);
+                        du[k] = act[k] ? d : 0;
+                    }
 #pragma unroll
                     for (uint32_t k = 0; k < kBU; ++k)
                         old[k] = du[k] > L ? __hip_atomic_fetch_sub(core + u[k], 1u, __ATOMIC_RELAXED, KC_AGENT) : 0;
