@@ -177,21 +177,28 @@ __device__ uint32_t wg_peel_graph(BatchState<BLOCK>& S, const uint32_t* off, con
                 S.tstart[tid] = s;
                 __syncthreads();
                 for (uint32_t jb = 0; jb < T; jb += BLOCK * kBU) {
+                    // kBU searches advanced step by step together (independent LDS reads per
+                    // step), then unconditional adjacency loads: inactive lanes take arc 0,
+                    // which exists since T > 0
                     bool act[kBU];
-                    uint32_t u[kBU];
+                    uint32_t u[kBU], jj[kBU], o[kBU];
 #pragma unroll
                     for (uint32_t k = 0; k < kBU; ++k) {
                         const uint32_t j = jb + k * BLOCK + tid;
                         act[k] = j < T;
-                        u[k] = 0;
-                        if (act[k]) {
-                            uint32_t o = 0;
+                        jj[k] = act[k] ? j : 0;
+                        o[k] = 0;
+                    }
 #pragma unroll
-                            for (uint32_t step = BLOCK / 2; step; step >>= 1)
-                                if (S.tincl[o + step - 1] <= j) o += step;
-                            const uint32_t ex = o ? S.tincl[o - 1] : 0;
-                            u[k] = nbr[S.tstart[o] + (j - ex)];
-                        }
+                    for (uint32_t step = BLOCK / 2; step; step >>= 1) {
+#pragma unroll
+                        for (uint32_t k = 0; k < kBU; ++k)
+                            if (S.tincl[o[k] + step - 1] <= jj[k]) o[k] += step;
+                    }
+#pragma unroll
+                    for (uint32_t k = 0; k < kBU; ++k) {
+                        const uint32_t ex = o[k] ? S.tincl[o[k] - 1] : 0;
+                        u[k] = nbr[S.tstart[o[k]] + (jj[k] - ex)];
                     }
                     // prechecks loaded unconditionally so they overlap (inactive lanes read
                     // the graph's first offset, read-only), as in the single-graph kernels

@@ -382,35 +382,52 @@ __device__ void process_vertices(const PeelArgs& a, WgState<BLOCK>& S, uint32_t 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    auto arc = [&](uint32_t j) -> uint32_t {
-        uint32_t k = 0;
+    // The neighbour at flattened positions j[0..kU): kU binary searches over the wave's
+    // inclusive prefix sums, advanced step by step together so each step's kU LDS reads are
+    // independent, then the kU adjacency loads.
+    auto arcs = [&](const uint32_t (&j)[kU], uint32_t (&out)[kU]) {
+        uint32_t at[kU];
 #pragma unroll
-        for (uint32_t step = 32; step; step >>= 1)
-            if (lin[k + step - 1] <= j) k += step;
-        const uint32_t excl = k ? lin[k - 1] : 0;
-        return a.nbr[lst[k] + (j - excl)];
+        for (uint32_t k = 0; k < kU; ++k) at[k] = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1) {
+#pragma unroll
+            for (uint32_t k = 0; k < kU; ++k)
+                if (lin[at[k] + step - 1] <= j[k]) at[k] += step;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kU; ++k) {
+            const uint32_t excl = at[k] ? lin[at[k] - 1] : 0;
+            out[k] = a.nbr[lst[at[k]] + (j[k] - excl)];
+        }
     };
+    // Adjacency loads are unconditional too (inactive lanes read arc 0, which exists since
+    // T > 0), so a group's kU loads leave back to back; whether a next group exists is the
+    // same for the whole wave.
     bool act[kU];
-    uint32_t u[kU];
+    uint32_t u[kU], j[kU];
 #pragma unroll
     for (uint32_t k = 0; k < kU; ++k) {
         act[k] = k * 64 + lane < T;
-        u[k] = act[k] ? arc(k * 64 + lane) : 0;
+        j[k] = act[k] ? k * 64 + lane : 0;
     }
+    arcs(j, u);
     for (uint32_t base = 0; base < T; base += 64 * kU) {
+        const bool more = base + 64 * kU < T;
         bool actn[kU];
         uint32_t un[kU];
 #pragma unroll
         for (uint32_t k = 0; k < kU; ++k) {
             const uint32_t jn = base + 64 * kU + k * 64 + lane;
             actn[k] = jn < T;
+            j[k] = actn[k] ? jn : 0;
             un[k] = 0;
-            if (PREFETCH && actn[k]) un[k] = arc(jn);  // next group's loads fly during this one's atomics
         }
+        if (PREFETCH && more) arcs(j, un);  // next group's loads fly during this one's atomics
         relax_group(a, S, L, act, u, mn, sn, local);
+        if (!PREFETCH && more) arcs(j, un);
 #pragma unroll
         for (uint32_t k = 0; k < kU; ++k) {
-            if (!PREFETCH && actn[k]) un[k] = arc(base + 64 * kU + k * 64 + lane);
             u[k] = un[k];
             act[k] = actn[k];
         }
@@ -423,14 +440,16 @@ __device__ void process_chunk(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, 
                               uint32_t& sn, bool local) {
     const uint32_t jb = (uint32_t)ch, je = (uint32_t)(ch >> 32);
     const uint32_t lane = lane_id();
+    // unconditional adjacency loads, as in process_vertices (inactive lanes read arc jb < je)
     bool act[kU];
     uint32_t u[kU];
 #pragma unroll
     for (uint32_t k = 0; k < kU; ++k) {
         act[k] = jb + k * 64 + lane < je;
-        u[k] = act[k] ? a.nbr[jb + k * 64 + lane] : 0;
+        u[k] = a.nbr[act[k] ? jb + k * 64 + lane : jb];
     }
     for (uint32_t base = jb; base < je; base += 64 * kU) {
+        const bool more = base + 64 * kU < je;
         bool actn[kU];
         uint32_t un[kU];
 #pragma unroll
@@ -438,12 +457,18 @@ __device__ void process_chunk(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L, 
             const uint32_t jn = base + 64 * kU + k * 64 + lane;
             actn[k] = jn < je;
             un[k] = 0;
-            if (PREFETCH && actn[k]) un[k] = a.nbr[jn];
+        }
+        if (PREFETCH && more) {
+#pragma unroll
+            for (uint32_t k = 0; k < kU; ++k) un[k] = a.nbr[actn[k] ? base + 64 * kU + k * 64 + lane : jb];
         }
         relax_group(a, S, L, act, u, mn, sn, local);
+        if (!PREFETCH && more) {
+#pragma unroll
+            for (uint32_t k = 0; k < kU; ++k) un[k] = a.nbr[actn[k] ? base + 64 * kU + k * 64 + lane : jb];
+        }
 #pragma unroll
         for (uint32_t k = 0; k < kU; ++k) {
-            if (!PREFETCH && actn[k]) un[k] = a.nbr[base + 64 * kU + k * 64 + lane];
             u[k] = un[k];
             act[k] = actn[k];
         }
