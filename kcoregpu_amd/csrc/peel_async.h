@@ -153,13 +153,20 @@ __device__ __forceinline__ void scan_list(const PeelArgs& a, WgStateA<BLOCK>& S,
     uint32_t sn = 0, vn = 0;
     for (uint32_t t = gw * 64 * U; t < len; t += GW * 64 * U) {
         uint32_t id[U], d[U];
+        // unconditional loads so each group of U leaves back to back: entries past the end read
+        // the list's first slot (the lists hold padV entries) and their gathers the read-only
+        // offsets; both are then replaced by the sentinel
 #pragma unroll
         for (uint32_t k = 0; k < U; ++k) {
             const uint32_t i = t + k * 64 + lane;
-            id[k] = i < len ? ld_rlx(list + i) : kSent;
+            const uint32_t x = ld_rlx(list + (i < len ? i : 0u));
+            id[k] = i < len ? x : kSent;
         }
 #pragma unroll
-        for (uint32_t k = 0; k < U; ++k) d[k] = id[k] != kSent ? ld_rlx(a.core + id[k]) : kSent;
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t x = ld_rlx(id[k] != kSent ? a.core + id[k] : a.off);
+            d[k] = id[k] != kSent ? x : kSent;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < U; ++k) {
             const bool valid = id[k] != kSent;
