@@ -333,10 +333,10 @@ template <int BLOCK>
 __device__ __forceinline__ void relax_group(const PeelArgs& a, WgState<BLOCK>& S, uint32_t L,
                                             const bool (&act)[kU], const uint32_t (&u)[kU],
                                             uint32_t& mn, uint32_t& sn, bool local) {
-    // The kU prechecks are loaded unconditionally, so they leave back to back with one wait
-    // between them and the decrements. (Under a per-arc branch the compiler waits for each
-    // load before issuing the next.) Inactive lanes read the graph's first offset, a read-only
-    // word, instead of a degree that decrements may be contending for.
+    // The kU prechecks are loaded unconditionally, so they leave back to back and each
+    // decrement waits only for its own load. (Under a per-arc branch the compiler waits for
+    // each load before issuing the next.) Inactive lanes read the graph's first offset, a
+    // read-only word, instead of a degree that decrements may be contending for.
     uint32_t du[kU];
 #pragma unroll
     for (uint32_t k = 0; k < kU; ++k) {
